@@ -1,0 +1,148 @@
+"""Benchmark: RK4 steps/s of the dns step on one MI355X.
+
+Workload (one "step" = one RK4 time step of main.c:334-422 over the whole
+grid): Taylor-Green vortex n=512, nu=1/3000, dt=5e-4, inputs resident in
+HBM before the timed region.  FP64 is the headline `value`; the mixed form
+is timed the same way and reported with its energy error against the FP64
+run after the same number of steps.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 512]
+
+Prints one JSON line.  `roofline` is for the kernel with the largest share
+of the step time: algorithmic bytes per launch (dns_amd/roofline.py)
+divided by its average launch duration from HIP events recorded on the
+library's own stream during the timed region.  `cpu_baseline` times the
+CPU oracle (this project's OpenMP C restatement of main.c) for a bounded
+sample of the same workload on this host.
+"""
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "RK4 steps/s at 512³ (FP64 vs mixed FP32 @ energy err≤1e-5), % HBM peak"
+
+
+def tgv(n):
+    """Taylor-Green vortex as tgv.py:14-27 of the reference builds it."""
+    dx = 2 * math.pi / n
+    i = np.arange(n)
+    s, c = np.sin(dx * i), np.cos(dx * i)
+    u = np.empty((3, n, n, n))
+    u[0] = s[:, None, None] * c[None, :, None] * c[None, None, :]
+    u[1] = (-c)[:, None, None] * s[None, :, None] * c[None, None, :]
+    u[2] = 0.0
+    return u
+
+
+def gpu_run(dns_amd, n, nu, dt, precision, u, warmup, steps):
+    from dns_amd import roofline
+    s = dns_amd.Solver(n, nu, dt, precision)
+    s.load_velocity(u)
+    if warmup:
+        s.step(warmup)
+    s.diagnostics()
+    s.set_kernel_timing(True)
+    t0 = time.perf_counter()
+    s.step(steps)
+    e, z = s.diagnostics()  # synchronises the library's stream
+    wall = time.perf_counter() - t0
+    timing = s.kernel_timing()
+    s.set_kernel_timing(False)
+    per_pass = roofline.summarize(timing, n, precision, steps)
+    kern_ms = sum(d["ms"] for d in per_pass.values())
+    s.close()
+    return {"steps_per_s": steps / wall, "ms_per_step": 1e3 * wall / steps, "energy": e, "enstrophy": z,
+            "kernel_ms_per_step": kern_ms / steps, "passes": per_pass}
+
+
+def cpu_baseline(n, nu, dt, u, steps):
+    """Times the CPU oracle's RK4 step (oracle/_build/liboracle.so)."""
+    path = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+    L = ctypes.CDLL(path)
+    L.orc_create.restype = ctypes.c_void_p
+    L.orc_create.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
+    L.orc_load.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    L.orc_step.argtypes = [ctypes.c_void_p, ctypes.c_long]
+    L.orc_destroy.argtypes = [ctypes.c_void_p]
+    p = L.orc_create(n, nu, dt)
+    L.orc_load(p, u.ctypes.data)
+    t0 = time.perf_counter()
+    L.orc_step(p, steps)
+    wall = time.perf_counter() - t0
+    L.orc_destroy(p)
+    return {"value": steps / wall, "unit": "steps/s", "cores": int(L.orc_threads()), "kind": "port",
+            "sample": f"{steps} RK4 step(s) of the same n={n} Taylor-Green workload, FP64, "
+                      f"{wall:.1f} s wall, OpenMP C restatement of main.c with its own FFT "
+                      f"(FFTW is not installed)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py: this build measures one GPU only; multi-GPU runs are out of scope "
+              "(see the sharding design in DESIGN.md)", file=sys.stderr)
+        return 2
+    import dns_amd
+    from dns_amd import roofline
+
+    n = a.n
+    nu, dt = (1.0 / 3000, 5e-4) if n == 512 else (1.0 / 1600, 1e-3)
+    u = tgv(n)
+    print(f"bench: n={n} fp64 warmup={a.warmup} steps={a.steps}", file=sys.stderr, flush=True)
+    r64 = gpu_run(dns_amd, n, nu, dt, "fp64", u, a.warmup, a.steps)
+    mixed = None
+    if not a.no_mixed:
+        print("bench: mixed", file=sys.stderr, flush=True)
+        rmx = gpu_run(dns_amd, n, nu, dt, "mixed", u, a.warmup, a.steps)
+        mixed = {"steps_per_s": rmx["steps_per_s"], "ms_per_step": rmx["ms_per_step"],
+                 "speedup_vs_fp64": rmx["steps_per_s"] / r64["steps_per_s"],
+                 "energy_rel_err_vs_fp64": abs(rmx["energy"] - r64["energy"]) / r64["energy"],
+                 "enstrophy_rel_err_vs_fp64": abs(rmx["enstrophy"] - r64["enstrophy"]) / r64["enstrophy"],
+                 "passes": {k: {"avg_ms": v["avg_ms"], "GBs": v["GBs"], "frac": v["frac"]}
+                            for k, v in rmx["passes"].items()}}
+    dom_name, dom = max(r64["passes"].items(), key=lambda kv: kv[1]["ms"])
+    step_b = roofline.step_bytes(n, "fp64")
+    line = {
+        "metric": METRIC, "value": r64["steps_per_s"], "unit": "steps/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": r64["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"Taylor-Green vortex n={n}, nu={nu:.6g}, dt={dt:g}, one RK4 step (FP64)",
+                   "n": n, "nu": nu, "dt": dt, "precision": "fp64"},
+        "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": dom["GBs"], "peak": roofline.PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": dom["frac"], "traffic": None,
+                     "bytes_per_launch": dom["bytes_per_launch"], "avg_launch_ms": dom["avg_ms"]},
+        "step_roofline": {"algorithmic_bytes_per_step": step_b,
+                          "achieved_GBs": step_b * r64["steps_per_s"] / 1e9,
+                          "frac": step_b * r64["steps_per_s"] / 1e9 / roofline.PEAK_HBM_GBS},
+        "kernels": {k: {"avg_ms": v["avg_ms"], "GBs": v["GBs"], "frac": v["frac"],
+                        "ms_per_step": v["ms_per_step"]} for k, v in r64["passes"].items()},
+        "mixed": mixed,
+        "energy": r64["energy"], "enstrophy": r64["enstrophy"],
+    }
+    if not a.no_cpu:
+        print(f"bench: cpu baseline n={n} steps={a.cpu_steps}", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline(n, nu, dt, u, a.cpu_steps)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,193 @@
+"""dns_amd: MI355X-native RK4 step of the pseudo-spectral Navier-Stokes
+solver `dns` (slitvinov/dns).
+
+The product is the C-ABI library ``libdns_hip.so`` (include/dns.h) and the
+``dns`` command line built on it.  This module is a thin ctypes binding of
+that library for Python callers (tests, benchmarks, torch autograd); it
+never falls back to any other implementation -- if the library is missing
+it raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = ["Solver", "DnsError", "lib", "LIB_PATH", "FP64", "MIXED", "abi_functions"]
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+LIB_PATH = os.path.join(PKG, "libdns_hip.so")
+HEADER = os.path.join(ROOT, "include", "dns.h")
+FP64, MIXED = 0, 1
+KERNEL_SLOTS = 20
+
+DNS_OK, DNS_EINVAL, DNS_ENODEV, DNS_ENOMEM, DNS_EHIP, DNS_ENONFINITE = 0, -1, -2, -3, -4, -5
+
+_lib = None
+
+
+class DnsError(RuntimeError):
+    def __init__(self, code, where):
+        self.code = code
+        detail = lib().dns_last_error().decode()
+        msg = f"{where}: {lib().dns_strerror(code).decode()} ({code})"
+        super().__init__(msg + (f": {detail}" if detail else ""))
+
+
+def lib():
+    """Load libdns_hip.so (built in-tree by ``python -m dns_amd.build``)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: run `python -m dns_amd.build` first")
+    L = ctypes.CDLL(LIB_PATH)
+    P, D, I, LL = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int, ctypes.c_long
+    sig = {
+        "dns_create": (I, [ctypes.POINTER(P), I, ctypes.c_double, ctypes.c_double, I, I]),
+        "dns_destroy": (None, [P]),
+        "dns_get_n": (I, [P]),
+        "dns_load_velocity": (I, [P, P]),
+        "dns_step": (I, [P, LL]),
+        "dns_set_pressure_capture": (I, [P, I]),
+        "dns_diagnostics": (I, [P, D, D]),
+        "dns_snapshot": (I, [P, P]),
+        "dns_get_spectral": (I, [P, P]),
+        "dns_set_spectral": (I, [P, P]),
+        "dns_get_stale_velocity": (I, [P, P]),
+        "dns_set_stale_velocity": (I, [P, P]),
+        "dns_rfft3": (I, [P, P, P]),
+        "dns_irfft3": (I, [P, P, P]),
+        "dns_device_bytes": (ctypes.c_longlong, [P]),
+        "dns_set_kernel_timing": (I, [P, I]),
+        "dns_kernel_timing": (I, [P, D, ctypes.POINTER(LL)]),
+        "dns_kernel_slot_name": (ctypes.c_char_p, [I]),
+        "dns_strerror": (ctypes.c_char_p, [I]),
+        "dns_last_error": (ctypes.c_char_p, []),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def abi_functions(header=HEADER):
+    """Names of the functions declared in include/dns.h."""
+    import re
+    text = open(header).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(dns_[a-z0-9_]+)\s*\(", text)))
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _check(rc, where):
+    if rc != DNS_OK:
+        raise DnsError(rc, where)
+
+
+class Solver:
+    """One solver context on one GPU (see include/dns.h)."""
+
+    def __init__(self, n, nu, dt, precision=FP64, device=0):
+        if isinstance(precision, str):
+            precision = {"fp64": FP64, "mixed": MIXED}[precision]
+        self.n, self.nu, self.dt, self.precision = int(n), float(nu), float(dt), precision
+        self._ctx = ctypes.c_void_p()
+        _check(lib().dns_create(ctypes.byref(self._ctx), self.n, self.nu, self.dt, precision, device),
+               "dns_create")
+
+    @property
+    def nf(self):
+        return self.n // 2 + 1
+
+    def close(self):
+        if self._ctx:
+            lib().dns_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def device_bytes(self):
+        return lib().dns_device_bytes(self._ctx)
+
+    def set_kernel_timing(self, on=True):
+        _check(lib().dns_set_kernel_timing(self._ctx, int(bool(on))), "dns_set_kernel_timing")
+
+    def kernel_timing(self):
+        """{slot name: (total ms, launches)} since timing was switched on."""
+        ms = (ctypes.c_double * KERNEL_SLOTS)()
+        cnt = (ctypes.c_long * KERNEL_SLOTS)()
+        _check(lib().dns_kernel_timing(self._ctx, ms, cnt), "dns_kernel_timing")
+        return {lib().dns_kernel_slot_name(i).decode(): (ms[i], cnt[i]) for i in range(KERNEL_SLOTS)}
+
+    def load_velocity(self, uvw):
+        uvw = np.ascontiguousarray(uvw, dtype=np.float64)
+        assert uvw.shape == (3, self.n, self.n, self.n) or uvw.size >= 3 * self.n ** 3
+        _check(lib().dns_load_velocity(self._ctx, _ptr(uvw)), "dns_load_velocity")
+
+    def step(self, nsteps=1):
+        _check(lib().dns_step(self._ctx, int(nsteps)), "dns_step")
+
+    def set_pressure_capture(self, on=True):
+        _check(lib().dns_set_pressure_capture(self._ctx, int(bool(on))), "dns_set_pressure_capture")
+
+    def diagnostics(self, check=True):
+        e, z = ctypes.c_double(), ctypes.c_double()
+        rc = lib().dns_diagnostics(self._ctx, ctypes.byref(e), ctypes.byref(z))
+        if check or rc != DNS_ENONFINITE:
+            _check(rc, "dns_diagnostics")
+        return e.value, z.value
+
+    def snapshot(self):
+        out = np.empty((4, self.n, self.n, self.n), dtype=np.float64)
+        _check(lib().dns_snapshot(self._ctx, _ptr(out)), "dns_snapshot")
+        return out
+
+    def get_spectral(self):
+        out = np.empty((3, self.n, self.n, self.nf), dtype=np.complex128)
+        _check(lib().dns_get_spectral(self._ctx, _ptr(out)), "dns_get_spectral")
+        return out
+
+    def set_spectral(self, a):
+        a = np.ascontiguousarray(a, dtype=np.complex128)
+        assert a.shape == (3, self.n, self.n, self.nf)
+        _check(lib().dns_set_spectral(self._ctx, _ptr(a)), "dns_set_spectral")
+
+    def get_stale_velocity(self):
+        out = np.empty((3, self.n, self.n, self.n), dtype=np.float64)
+        _check(lib().dns_get_stale_velocity(self._ctx, _ptr(out)), "dns_get_stale_velocity")
+        return out
+
+    def set_stale_velocity(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.shape == (3, self.n, self.n, self.n)
+        _check(lib().dns_set_stale_velocity(self._ctx, _ptr(a)), "dns_set_stale_velocity")
+
+    def rfft3(self, real):
+        real = np.ascontiguousarray(real, dtype=np.float64)
+        assert real.shape == (self.n,) * 3
+        out = np.empty((self.n, self.n, self.nf), dtype=np.complex128)
+        _check(lib().dns_rfft3(self._ctx, _ptr(real), _ptr(out)), "dns_rfft3")
+        return out
+
+    def irfft3(self, spec):
+        spec = np.ascontiguousarray(spec, dtype=np.complex128)
+        assert spec.shape == (self.n, self.n, self.nf)
+        out = np.empty((self.n,) * 3, dtype=np.float64)
+        _check(lib().dns_irfft3(self._ctx, _ptr(spec), _ptr(out)), "dns_irfft3")
+        return out

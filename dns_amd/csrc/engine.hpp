@@ -1,0 +1,368 @@
+// Context and per-grid-size engine of libdns_hip.
+//
+// Device memory plan (U = one spectral field, n*n*nfp complex; R = one
+// physical field, n^3 reals):
+//   uhn  3U double  spectral state at the start of the step (Û_n); stage 4
+//                   writes Û_{n+1} here
+//   uhr  3U SR      stage state Û_r (the argument of stages 2..4)
+//   acc  3U double  RK4 accumulator (Û1 of main.c)
+//   W    6U R       FFT partials: u,v,w in fields 0..2, omega in 3..5;
+//                   N = u x omega in 0..2 after the fused z pass.  Also
+//                   the scratch of the initial transform and of snapshots.
+//   us   3R R       stale physical velocity (main.c keeps U,V,W from the
+//                   previous stage 4 for stage 1, main.c:344-353)
+//   phat 1U double  stage-4 pressure, allocated on first capture request
+// FP64: R = SR = double.  Mixed: R = SR = float.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+#include "../../include/dns.h"
+#include "kernels.hpp"
+
+namespace dnsk {
+
+void set_error(const char *what, hipError_t e);
+
+#define DNS_HIP(x)                                                   \
+  do {                                                               \
+    hipError_t e_ = (x);                                             \
+    if (e_ != hipSuccess) {                                          \
+      ::dnsk::set_error(#x, e_);                                     \
+      return e_ == hipErrorOutOfMemory ? DNS_ENOMEM : DNS_EHIP;      \
+    }                                                                \
+  } while (0)
+
+#define DNS_LAUNCHED() DNS_HIP(hipGetLastError())
+
+struct Impl;
+
+}  // namespace dnsk
+
+// Kernel timing slots: 5 * stage + pass, pass A..E = 0..4 (see kernels.hpp).
+enum { DNS_NSLOT = 20 };
+
+struct dns_ctx {
+  int n, nf, nfp, prec, device;
+  double nu, dt, kmax, invn3;
+  hipStream_t stream;
+  dnsk::Geo g;
+  void *uhn, *uhr, *acc, *W, *us, *phat;
+  void *tw64, *tw32;
+  double *part, *dres;
+  int capture, have_phat;
+  long long bytes;
+  dnsk::Impl *impl;
+  // HIP-event timing of the step's kernels on `stream` (off by default)
+  int timing;
+  std::vector<hipEvent_t> ev_pool;
+  struct TimedLaunch {
+    int slot;
+    hipEvent_t a, b;
+  };
+  std::vector<TimedLaunch> ev_pending;
+  double slot_ms[DNS_NSLOT];
+  long slot_n[DNS_NSLOT];
+};
+
+namespace dnsk {
+
+struct Impl {
+  virtual ~Impl() {}
+  virtual size_t sr_size() const = 0;  // bytes per complex element of uhr
+  virtual size_t r_size() const = 0;   // bytes per real element of W/us
+  virtual int make_twiddles(dns_ctx *c) = 0;
+  virtual int load(dns_ctx *c, const double *uvw) = 0;
+  virtual int step(dns_ctx *c, long nsteps) = 0;
+  virtual int snapshot(dns_ctx *c, double *uvwp) = 0;
+  virtual int rfft3(dns_ctx *c, const double *in, double *out) = 0;
+  virtual int irfft3(dns_ctx *c, const double *in, double *out) = 0;
+  virtual int get_us(dns_ctx *c, double *out) = 0;
+  virtual int set_us(dns_ctx *c, const double *in) = 0;
+};
+
+// Event bracketing of one launch when timing is on.
+inline hipEvent_t ev_get(dns_ctx *c) {
+  if (!c->ev_pool.empty()) {
+    hipEvent_t e = c->ev_pool.back();
+    c->ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+inline void t_begin(dns_ctx *c, int slot) {
+  if (!c->timing) return;
+  dns_ctx::TimedLaunch t{slot, ev_get(c), ev_get(c)};
+  if (t.a) hipEventRecord(t.a, c->stream);
+  c->ev_pending.push_back(t);
+}
+inline void t_end(dns_ctx *c) {
+  if (!c->timing || c->ev_pending.empty()) return;
+  hipEvent_t b = c->ev_pending.back().b;
+  if (b) hipEventRecord(b, c->stream);
+}
+#define DNS_TIMED(c, slot, launch) \
+  do {                             \
+    t_begin(c, slot);              \
+    launch;                        \
+    t_end(c);                      \
+    DNS_LAUNCHED();                \
+  } while (0)
+
+template <typename TI, typename TO>
+__global__ void k_convert(const TI *__restrict__ in, TO *__restrict__ out, long count) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < count; i += (long)gridDim.x * blockDim.x)
+    out[i] = (TO)in[i];
+}
+
+// Forward twiddles of the radix schedule of N: per stage s, entries
+// k*(r-1) + m-1 = exp(-2 pi i m k / (Ns r)), computed in long double.
+template <int N> std::vector<long double> twiddles_ld() {
+  std::vector<long double> t(2 * tw_size<N>() + 2, 0.0L);
+  const long double PI = 3.141592653589793238462643383279502884L;
+  int off = 0, Ns = 1;
+  for (int s = 0; s < Sched<N>::NS; s++) {
+    const int r = Sched<N>::r[s];
+    for (int k = 0; k < Ns; k++)
+      for (int m = 1; m < r; m++) {
+        const long double a = 2.0L * PI * (long double)(m * k) / (long double)(Ns * r);
+        t[2 * (off + k * (r - 1) + m - 1)] = cosl(a);
+        t[2 * (off + k * (r - 1) + m - 1) + 1] = -sinl(a);
+      }
+    off += Ns * (r - 1);
+    Ns *= r;
+  }
+  return t;
+}
+
+template <int N, typename R, typename SR> struct Engine final : Impl {
+  static constexpr int E = Sched<N>::E, TPP = N / E;
+  // Column-tile widths: 128-byte row segments, at most 1024 threads.
+  static constexpr int T_R = (sizeof(R) == 8 ? 8 : 16) * TPP > 1024 ? 1024 / TPP : (sizeof(R) == 8 ? 8 : 16);
+  static constexpr int T_D = 8 * TPP > 1024 ? 1024 / TPP : 8;
+  // z pencils per block: about 256 threads.
+  static constexpr int PPW = TPP >= 256 ? 1 : 256 / TPP;
+  static constexpr int NF = N / 2 + 1;
+
+  size_t sr_size() const override { return sizeof(cx<SR>); }
+  size_t r_size() const override { return sizeof(R); }
+
+  static const cx<R> *twR(dns_ctx *c) { return (const cx<R> *)(sizeof(R) == 8 ? c->tw64 : c->tw32); }
+  static const cx<double> *twD(dns_ctx *c) { return (const cx<double> *)c->tw64; }
+
+  dim3 grid_x(int T) const { return dim3((NF + T - 1) / T, N); }
+
+  int make_twiddles(dns_ctx *c) override {
+    auto t = twiddles_ld<N>();
+    std::vector<double> d(t.size());
+    std::vector<float> f(t.size());
+    for (size_t i = 0; i < t.size(); i++) {
+      d[i] = (double)t[i];
+      f[i] = (float)t[i];  // single rounding from the long double value
+    }
+    DNS_HIP(hipMalloc(&c->tw64, d.size() * sizeof(double)));
+    DNS_HIP(hipMemcpy(c->tw64, d.data(), d.size() * sizeof(double), hipMemcpyHostToDevice));
+    DNS_HIP(hipMalloc(&c->tw32, f.size() * sizeof(float)));
+    DNS_HIP(hipMemcpy(c->tw32, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
+    c->bytes += (long long)(d.size() * sizeof(double) + f.size() * sizeof(float));
+    return DNS_OK;
+  }
+
+  // Forward 3-D transform (double) of `nf` physical fields at `phys` into
+  // spectral fields at `spec` (in the padded layout).
+  int fwd3_double(dns_ctx *c, const double *phys, double2 *spec, int nfields) {
+    hipStream_t s = c->stream;
+    k_zr2c<N, double, PPW><<<N * N / PPW, PPW * TPP, 0, s>>>(phys, spec, nfields, c->g, twD(c));
+    DNS_LAUNCHED();
+    k_ypass<N, double, T_D, false><<<grid_x(T_D), T_D * TPP, 0, s>>>(spec, nfields, c->g, twD(c));
+    DNS_LAUNCHED();
+    k_xplain<N, double, T_D, false, double><<<grid_x(T_D), T_D * TPP, 0, s>>>(spec, spec, nfields, c->g, twD(c));
+    DNS_LAUNCHED();
+    return DNS_OK;
+  }
+
+  int load(dns_ctx *c, const double *uvw) override {
+    hipStream_t s = c->stream;
+    double *tmp = (double *)c->W;  // 3 physical double fields fit in W
+    const size_t rb = sizeof(double) * c->g.rs;
+    DNS_HIP(hipMemcpyAsync(tmp, uvw, 3 * rb, hipMemcpyHostToDevice, s));
+    int rc = fwd3_double(c, tmp, (double2 *)c->uhn, 3);
+    if (rc) return rc;
+    if (sizeof(R) == 8) {
+      DNS_HIP(hipMemcpyAsync(c->us, tmp, 3 * rb, hipMemcpyDeviceToDevice, s));
+    } else {
+      k_convert<double, R><<<4096, 256, 0, s>>>(tmp, (R *)c->us, 3 * c->g.rs);
+      DNS_LAUNCHED();
+    }
+    DNS_HIP(hipStreamSynchronize(s));
+    return DNS_OK;
+  }
+
+  int step(dns_ctx *c, long nsteps) override {
+    hipStream_t s = c->stream;
+    const Geo g = c->g;
+    cx<R> *W = (cx<R> *)c->W;
+    cx<SR> *uhr = (cx<SR> *)c->uhr;
+    double2 *uhn = (double2 *)c->uhn, *acc = (double2 *)c->acc;
+    const cx<R> *tw = twR(c);
+    const R invn3 = (R)c->invn3;
+    static const double a_rk[4] = {1 / 6.0, 1 / 3.0, 1 / 3.0, 1 / 6.0};  // main.c:15
+    static const double b_rk[4] = {0.5, 0.5, 1.0, 0.0};                 // main.c:16
+    const dim3 gx = grid_x(T_R), bx(T_R * TPP);
+    const dim3 gz(N * N / PPW), bz(PPW * TPP);
+    for (long st = 0; st < nsteps; st++) {
+      double2 *ph = (c->capture && st == nsteps - 1) ? (double2 *)c->phat : nullptr;
+      for (int rk = 0; rk < 4; rk++) {
+        RKParams q{c->dt, c->nu, c->kmax, a_rk[rk], b_rk[rk]};
+        const int sl = 5 * rk;
+        if (rk == 0) {
+          DNS_TIMED(c, sl + 0, (k_xinv_curl<N, R, T_R, false, double><<<gx, bx, 0, s>>>(uhn, W, g, tw)));
+          DNS_TIMED(c, sl + 1, (k_ypass<N, R, T_R, true><<<gx, bx, 0, s>>>(W + 3 * g.fs, 3, g, tw)));
+          DNS_TIMED(c, sl + 2, (k_zfused<N, R, PPW, 0, R><<<gz, bz, 0, s>>>(W, (R *)c->us, g, invn3, tw)));
+        } else {
+          DNS_TIMED(c, sl + 0, (k_xinv_curl<N, R, T_R, true, SR><<<gx, bx, 0, s>>>(uhr, W, g, tw)));
+          DNS_TIMED(c, sl + 1, (k_ypass<N, R, T_R, true><<<gx, bx, 0, s>>>(W, 6, g, tw)));
+          if (rk < 3)
+            DNS_TIMED(c, sl + 2, (k_zfused<N, R, PPW, 1, R><<<gz, bz, 0, s>>>(W, (R *)c->us, g, invn3, tw)));
+          else
+            DNS_TIMED(c, sl + 2, (k_zfused<N, R, PPW, 2, R><<<gz, bz, 0, s>>>(W, (R *)c->us, g, invn3, tw)));
+        }
+        DNS_TIMED(c, sl + 3, (k_ypass<N, R, T_R, false><<<gx, bx, 0, s>>>(W, 3, g, tw)));
+        switch (rk) {
+          case 0:
+            DNS_TIMED(c, sl + 4, (k_xfwd_rk<N, R, T_R, 0, SR><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, nullptr, g, q, tw)));
+            break;
+          case 1:
+            DNS_TIMED(c, sl + 4, (k_xfwd_rk<N, R, T_R, 1, SR><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, nullptr, g, q, tw)));
+            break;
+          case 2:
+            DNS_TIMED(c, sl + 4, (k_xfwd_rk<N, R, T_R, 2, SR><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, nullptr, g, q, tw)));
+            break;
+          default:
+            DNS_TIMED(c, sl + 4, (k_xfwd_rk<N, R, T_R, 3, SR><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, ph, g, q, tw)));
+            break;
+        }
+      }
+      if (ph) c->have_phat = 1;
+    }
+    return DNS_OK;
+  }
+
+  // c2r (double) of one spectral double field into a physical double
+  // field times `scale`; uses W as scratch beyond `phys_out` if needed.
+  int inv3_double(dns_ctx *c, const double2 *spec, double2 *tmpc, double *phys, double scale) {
+    hipStream_t s = c->stream;
+    k_xplain<N, double, T_D, true, double><<<grid_x(T_D), T_D * TPP, 0, s>>>(spec, tmpc, 1, c->g, twD(c));
+    DNS_LAUNCHED();
+    k_ypass<N, double, T_D, true><<<grid_x(T_D), T_D * TPP, 0, s>>>(tmpc, 1, c->g, twD(c));
+    DNS_LAUNCHED();
+    k_zc2r<N, double, PPW><<<N * N / PPW, PPW * TPP, 0, s>>>(tmpc, phys, c->g, scale, twD(c));
+    DNS_LAUNCHED();
+    return DNS_OK;
+  }
+
+  int snapshot(dns_ctx *c, double *uvwp) override {
+    hipStream_t s = c->stream;
+    double2 *tmpc = (double2 *)c->W;
+    double *tmpr = (double *)((char *)c->W + sizeof(double2) * c->g.fs);
+    const size_t rb = sizeof(double) * c->g.rs;
+    for (int f = 0; f < 4; f++) {
+      if (f == 3 && !c->have_phat) {
+        memset(uvwp + 3 * c->g.rs, 0, rb);
+        continue;
+      }
+      const double2 *src = f < 3 ? (const double2 *)c->uhn + f * c->g.fs : (const double2 *)c->phat;
+      int rc = inv3_double(c, src, tmpc, tmpr, c->invn3);
+      if (rc) return rc;
+      DNS_HIP(hipMemcpyAsync(uvwp + f * c->g.rs, tmpr, rb, hipMemcpyDeviceToHost, s));
+      DNS_HIP(hipStreamSynchronize(s));
+    }
+    return DNS_OK;
+  }
+
+  int rfft3(dns_ctx *c, const double *in, double *out) override {
+    hipStream_t s = c->stream;
+    const Geo g = c->g;
+    double *phys = (double *)c->W;
+    cx<R> *spec = (cx<R> *)((char *)c->W + sizeof(double) * g.rs);
+    DNS_HIP(hipMemcpyAsync(phys, in, sizeof(double) * g.rs, hipMemcpyHostToDevice, s));
+    k_zr2c<N, R, PPW><<<N * N / PPW, PPW * TPP, 0, s>>>(phys, spec, 1, g, twR(c));
+    DNS_LAUNCHED();
+    k_ypass<N, R, T_R, false><<<grid_x(T_R), T_R * TPP, 0, s>>>(spec, 1, g, twR(c));
+    DNS_LAUNCHED();
+    k_xplain<N, R, T_R, false, R><<<grid_x(T_R), T_R * TPP, 0, s>>>(spec, spec, 1, g, twR(c));
+    DNS_LAUNCHED();
+    double2 *specd = (double2 *)spec;
+    if (sizeof(R) != 8) {  // widen into the region after the float spectrum
+      specd = (double2 *)((char *)spec + sizeof(cx<R>) * g.fs);
+      k_convert<R, double><<<4096, 256, 0, s>>>((const R *)spec, (double *)specd, 2 * g.fs);
+      DNS_LAUNCHED();
+    }
+    DNS_HIP(hipMemcpy2DAsync(out, sizeof(double2) * NF, specd, sizeof(double2) * g.nfp, sizeof(double2) * NF,
+                             (size_t)N * N, hipMemcpyDeviceToHost, s));
+    DNS_HIP(hipStreamSynchronize(s));
+    return DNS_OK;
+  }
+
+  int irfft3(dns_ctx *c, const double *in, double *out) override {
+    hipStream_t s = c->stream;
+    const Geo g = c->g;
+    // W: [spectral double input | complex R partial | physical double output]
+    double2 *specd = (double2 *)c->W;
+    cx<R> *tmpc = (cx<R> *)((char *)c->W + sizeof(double2) * g.fs);
+    double *phys = (double *)((char *)tmpc + sizeof(cx<R>) * g.fs);
+    DNS_HIP(hipMemset2DAsync(specd, sizeof(double2) * g.nfp, 0, sizeof(double2) * g.nfp, (size_t)N * N, s));
+    DNS_HIP(hipMemcpy2DAsync(specd, sizeof(double2) * g.nfp, in, sizeof(double2) * NF, sizeof(double2) * NF,
+                             (size_t)N * N, hipMemcpyHostToDevice, s));
+    k_xplain<N, R, T_R, true, double><<<grid_x(T_R), T_R * TPP, 0, s>>>(specd, tmpc, 1, g, twR(c));
+    DNS_LAUNCHED();
+    k_ypass<N, R, T_R, true><<<grid_x(T_R), T_R * TPP, 0, s>>>(tmpc, 1, g, twR(c));
+    DNS_LAUNCHED();
+    k_zc2r<N, R, PPW><<<N * N / PPW, PPW * TPP, 0, s>>>(tmpc, phys, g, (R)1, twR(c));
+    DNS_LAUNCHED();
+    DNS_HIP(hipMemcpyAsync(out, phys, sizeof(double) * g.rs, hipMemcpyDeviceToHost, s));
+    DNS_HIP(hipStreamSynchronize(s));
+    return DNS_OK;
+  }
+
+  int get_us(dns_ctx *c, double *out) override {
+    hipStream_t s = c->stream;
+    const size_t cnt = 3 * (size_t)c->g.rs;
+    const double *src = (const double *)c->us;
+    if (sizeof(R) != 8) {
+      double *tmp = (double *)c->W;
+      k_convert<R, double><<<4096, 256, 0, s>>>((const R *)c->us, tmp, (long)cnt);
+      DNS_LAUNCHED();
+      src = tmp;
+    }
+    DNS_HIP(hipMemcpyAsync(out, src, cnt * sizeof(double), hipMemcpyDeviceToHost, s));
+    DNS_HIP(hipStreamSynchronize(s));
+    return DNS_OK;
+  }
+
+  int set_us(dns_ctx *c, const double *in) override {
+    hipStream_t s = c->stream;
+    const size_t cnt = 3 * (size_t)c->g.rs;
+    if (sizeof(R) == 8) {
+      DNS_HIP(hipMemcpyAsync(c->us, in, cnt * sizeof(double), hipMemcpyHostToDevice, s));
+    } else {
+      double *tmp = (double *)c->W;
+      DNS_HIP(hipMemcpyAsync(tmp, in, cnt * sizeof(double), hipMemcpyHostToDevice, s));
+      k_convert<double, R><<<4096, 256, 0, s>>>(tmp, (R *)c->us, (long)cnt);
+      DNS_LAUNCHED();
+    }
+    DNS_HIP(hipStreamSynchronize(s));
+    return DNS_OK;
+  }
+};
+
+template <int N> Impl *make_engine(int prec) {
+  if (prec == DNS_MIXED) return new Engine<N, float, float>();
+  return new Engine<N, double, double>();
+}
+
+}  // namespace dnsk

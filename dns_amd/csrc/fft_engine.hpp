@@ -1,0 +1,201 @@
+// Device-side Stockham FFT engine for gfx950 (CDNA4).
+//
+// A length-N pencil is owned by TPP = N/E threads; thread t holds the E
+// elements at positions t + TPP*c (c = 0..E-1) in registers.  Each stage of
+// radix r performs E/r butterflies per thread:
+//   butterfly j = t + q*TPP reads x[j + m*N/r] (= register q + m*E/r),
+//   multiplies by w^(m*(j mod Ns)) with w = exp(-+2 pi i/(Ns r)), applies a
+//   radix-r DFT and writes y[(j/Ns)*Ns*r + j%Ns + m*Ns].
+// Between stages the values go through LDS; after the last stage
+// (Ns = N/r) the outputs land back on the load positions t + TPP*c, so a
+// pencil is read from and written to HBM with the same, coalesced pattern.
+//
+// Twiddles come from a per-N table (generated on the host in long double,
+// rounded once to the working precision); stage 0 needs none.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <utility>
+
+namespace dnsk {
+
+template <typename R> struct cx_of;
+template <> struct cx_of<double> { using type = double2; };
+template <> struct cx_of<float> { using type = float2; };
+template <typename R> using cx = typename cx_of<R>::type;
+
+template <typename R> __device__ __forceinline__ cx<R> mk(R a, R b) {
+  cx<R> z;
+  z.x = a;
+  z.y = b;
+  return z;
+}
+template <typename R> __device__ __forceinline__ cx<R> cadd(cx<R> a, cx<R> b) { return mk<R>(a.x + b.x, a.y + b.y); }
+template <typename R> __device__ __forceinline__ cx<R> csub(cx<R> a, cx<R> b) { return mk<R>(a.x - b.x, a.y - b.y); }
+template <typename R> __device__ __forceinline__ cx<R> cmul(cx<R> a, cx<R> b) {
+  return mk<R>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename R> __device__ __forceinline__ cx<R> cmulc(cx<R> a, cx<R> b) {  // a * conj(b)
+  return mk<R>(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+// multiply by -i (forward) or +i (inverse)
+template <typename R, bool INV> __device__ __forceinline__ cx<R> rot(cx<R> a) {
+  return INV ? mk<R>(-a.y, a.x) : mk<R>(a.y, -a.x);
+}
+
+// ---------------------------------------------------------------------------
+// Radix schedules.  E = elements per thread (divisible by every radix).
+template <int N> struct Sched;
+template <> struct Sched<16> { static constexpr int E = 4, NS = 2; static constexpr int r[NS] = {4, 4}; };
+template <> struct Sched<32> { static constexpr int E = 8, NS = 2; static constexpr int r[NS] = {8, 4}; };
+template <> struct Sched<48> { static constexpr int E = 12, NS = 3; static constexpr int r[NS] = {4, 4, 3}; };
+template <> struct Sched<64> { static constexpr int E = 8, NS = 2; static constexpr int r[NS] = {8, 8}; };
+template <> struct Sched<96> { static constexpr int E = 12, NS = 4; static constexpr int r[NS] = {4, 4, 2, 3}; };
+template <> struct Sched<128> { static constexpr int E = 8, NS = 3; static constexpr int r[NS] = {8, 4, 4}; };
+template <> struct Sched<192> { static constexpr int E = 12, NS = 4; static constexpr int r[NS] = {4, 4, 4, 3}; };
+template <> struct Sched<256> { static constexpr int E = 8, NS = 3; static constexpr int r[NS] = {8, 8, 4}; };
+template <> struct Sched<384> { static constexpr int E = 12, NS = 5; static constexpr int r[NS] = {4, 4, 4, 2, 3}; };
+template <> struct Sched<512> { static constexpr int E = 8, NS = 3; static constexpr int r[NS] = {8, 8, 8}; };
+template <> struct Sched<768> { static constexpr int E = 12, NS = 5; static constexpr int r[NS] = {4, 4, 4, 4, 3}; };
+template <> struct Sched<1024> { static constexpr int E = 8, NS = 4; static constexpr int r[NS] = {8, 8, 4, 4}; };
+
+template <int N> constexpr int stage_Ns(int s) {
+  int ns = 1;
+  for (int q = 0; q < s; q++) ns *= Sched<N>::r[q];
+  return ns;
+}
+template <int N> constexpr int stage_twoff(int s) {
+  int off = 0, ns = 1;
+  for (int q = 0; q < s; q++) {
+    off += ns * (Sched<N>::r[q] - 1);
+    ns *= Sched<N>::r[q];
+  }
+  return off;
+}
+template <int N> constexpr int tw_size() { return stage_twoff<N>(Sched<N>::NS); }
+
+template <int I, int END, class F> __device__ __forceinline__ void static_for(F &&f) {
+  if constexpr (I < END) {
+    f(std::integral_constant<int, I>{});
+    static_for<I + 1, END>(f);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Radix-r DFTs in registers (unnormalised; INV selects exp(+i)).
+template <typename R, bool INV> __device__ __forceinline__ void dft2(cx<R> *a) {
+  cx<R> t = a[0];
+  a[0] = cadd<R>(t, a[1]);
+  a[1] = csub<R>(t, a[1]);
+}
+template <typename R, bool INV> __device__ __forceinline__ void dft4(cx<R> *a) {
+  cx<R> s0 = cadd<R>(a[0], a[2]), d0 = csub<R>(a[0], a[2]);
+  cx<R> s1 = cadd<R>(a[1], a[3]), d1 = rot<R, INV>(csub<R>(a[1], a[3]));
+  a[0] = cadd<R>(s0, s1);
+  a[2] = csub<R>(s0, s1);
+  a[1] = cadd<R>(d0, d1);
+  a[3] = csub<R>(d0, d1);
+}
+template <typename R, bool INV> __device__ __forceinline__ void dft3(cx<R> *a) {
+  const R s3 = (R)0.86602540378443864676372317075294;  // sin(2 pi / 3)
+  cx<R> t1 = cadd<R>(a[1], a[2]);
+  cx<R> t2 = mk<R>(a[0].x - (R)0.5 * t1.x, a[0].y - (R)0.5 * t1.y);
+  cx<R> t3 = rot<R, INV>(mk<R>(s3 * (a[1].x - a[2].x), s3 * (a[1].y - a[2].y)));
+  a[0] = cadd<R>(a[0], t1);
+  a[1] = cadd<R>(t2, t3);
+  a[2] = csub<R>(t2, t3);
+}
+template <typename R, bool INV> __device__ __forceinline__ void dft8(cx<R> *a) {
+  const R h = (R)0.70710678118654752440084436210485;  // 1/sqrt(2)
+  cx<R> s[4], d[4];
+#pragma unroll
+  for (int m = 0; m < 4; m++) {
+    s[m] = cadd<R>(a[m], a[m + 4]);
+    d[m] = csub<R>(a[m], a[m + 4]);
+  }
+  // d[m] *= w8^m, w8 = exp(-+ i pi/4)
+  if (INV) {
+    d[1] = mk<R>(h * (d[1].x - d[1].y), h * (d[1].x + d[1].y));
+    d[2] = mk<R>(-d[2].y, d[2].x);
+    d[3] = mk<R>(-h * (d[3].x + d[3].y), h * (d[3].x - d[3].y));
+  } else {
+    d[1] = mk<R>(h * (d[1].x + d[1].y), h * (d[1].y - d[1].x));
+    d[2] = mk<R>(d[2].y, -d[2].x);
+    d[3] = mk<R>(h * (d[3].y - d[3].x), -h * (d[3].x + d[3].y));
+  }
+  dft4<R, INV>(s);
+  dft4<R, INV>(d);
+#pragma unroll
+  for (int p = 0; p < 4; p++) {
+    a[2 * p] = s[p];
+    a[2 * p + 1] = d[p];
+  }
+}
+template <typename R, int RAD, bool INV> __device__ __forceinline__ void dft(cx<R> *a) {
+  if constexpr (RAD == 2) dft2<R, INV>(a);
+  else if constexpr (RAD == 3) dft3<R, INV>(a);
+  else if constexpr (RAD == 4) dft4<R, INV>(a);
+  else dft8<R, INV>(a);
+}
+
+// ---------------------------------------------------------------------------
+// LDS slot maps.  One dummy row every 8 positions breaks the power-of-two
+// strides of the first stages' scattered writes.
+template <int N> __device__ __forceinline__ int padpos(int pos) { return pos + (pos >> 3); }
+template <int N> constexpr int padlen() { return N + N / 8; }
+
+// Tile of T pencils, pencil index fastest (strided-axis passes).
+template <int N, int T> struct TileSlot {
+  int p;
+  __device__ __forceinline__ int operator()(int pos) const { return padpos<N>(pos) * T + p; }
+};
+// Pencil-major (contiguous-axis passes).
+template <int N> struct PencilSlot {
+  int base;
+  __device__ __forceinline__ int operator()(int pos) const { return base + padpos<N>(pos); }
+};
+
+// In-register FFT of one pencil.  v[c] holds position t + TPP*c on entry
+// and the transform's coefficient at the same position on exit.
+// All threads of the block must call it together (it uses __syncthreads).
+template <int N, typename R, bool INV, class Slot>
+__device__ __forceinline__ void fft_pencil(cx<R> (&v)[Sched<N>::E], int t, cx<R> *lds, Slot slot,
+                                           const cx<R> *__restrict__ tw) {
+  constexpr int E = Sched<N>::E, NS = Sched<N>::NS, TPP = N / E;
+  static_for<0, NS>([&](auto S) {
+    constexpr int s = decltype(S)::value;
+    constexpr int r = Sched<N>::r[s];
+    constexpr int Ns = stage_Ns<N>(s);
+    constexpr int toff = stage_twoff<N>(s);
+    constexpr int Q = E / r;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+      const int j = t + q * TPP;
+      cx<R> a[r];
+#pragma unroll
+      for (int m = 0; m < r; m++) a[m] = v[q + m * Q];
+      if constexpr (s > 0) {
+        const int k = j % Ns;
+        const cx<R> *w = tw + toff + k * (r - 1);
+#pragma unroll
+        for (int m = 1; m < r; m++) a[m] = INV ? cmulc<R>(a[m], w[m - 1]) : cmul<R>(a[m], w[m - 1]);
+      }
+      dft<R, r, INV>(a);
+      if constexpr (s == NS - 1) {
+#pragma unroll
+        for (int m = 0; m < r; m++) v[q + m * Q] = a[m];
+      } else {
+        const int base = (j / Ns) * Ns * r + (j % Ns);
+#pragma unroll
+        for (int m = 0; m < r; m++) lds[slot(base + m * Ns)] = a[m];
+      }
+    }
+    if constexpr (s < NS - 1) {
+      __syncthreads();
+#pragma unroll
+      for (int c = 0; c < E; c++) v[c] = lds[slot(t + TPP * c)];
+      __syncthreads();
+    }
+  });
+}
+
+}  // namespace dnsk

@@ -1,0 +1,514 @@
+// HIP kernels of the RK4 step (gfx950).  All arrays are device-resident.
+//
+// Layouts (i = x slowest, j = y, k = z fastest, as in main.c:236,358):
+//   spectral field  : complex [n][n][nfp], element (i,j,k) at (i*n+j)*nfp+k,
+//                     k < nf = n/2+1 valid, nfp = nf rounded up to 16
+//                     (rows start on 128-byte boundaries);
+//   physical field  : real [n][n][n], (i*n+j)*n+k.
+// A "pencil" is the set of elements along one axis with the other two
+// indices fixed.  x and y pencils are strided: a block owns T pencils with
+// consecutive k (a T-wide column tile), so every wave load/store touches
+// whole T*sizeof(complex)-byte row segments.  z pencils are contiguous rows.
+//
+// Per RK stage the host launches (reference lines in main.c):
+//   A  k_xinv_curl : curl prologue (354-362) + inverse FFT along x
+//   B  k_ypass     : inverse FFT along y (in place)
+//   C  k_zfused    : c2r along z of u and omega, 1/n^3 scaling (348-352,
+//                    366-371), u x omega (372-377), r2c along z of N
+//   D  k_ypass     : forward FFT along y
+//   E  k_xfwd_rk   : forward FFT along x + dealias*dt (381-386), projection
+//                    and viscous term (387-399), RK update (400-413)
+#pragma once
+#include "fft_engine.hpp"
+
+namespace dnsk {
+
+struct Geo {
+  int nfp;   // padded row length of spectral fields (complex elements)
+  long fs;   // spectral field stride (complex elements) = n*n*nfp
+  long rs;   // physical field stride (reals) = n^3
+};
+
+struct RKParams {
+  double dt, nu, kmax;
+  double a, b;  // a[rk], b[rk] of main.c:15-16 for this stage
+};
+
+template <typename R> __device__ __forceinline__ cx<R> zero() { return mk<R>((R)0, (R)0); }
+
+template <typename RO, typename RI> __device__ __forceinline__ cx<RO> cvt(cx<RI> a) {
+  return mk<RO>((RO)a.x, (RO)a.y);
+}
+
+// Signed wavenumber of index i on an x or y axis: i for i < n/2, i - n
+// otherwise (main.c:224-230, Nyquist = -n/2).
+template <int N> __device__ __forceinline__ int wave(int i) { return i < N / 2 ? i : i - N; }
+
+// ---------------------------------------------------------------------------
+// B/D: FFT along y of `nfields` consecutive spectral fields, in place.
+// Grid (ceil(nf/T), n): blockIdx.y = i.
+template <int N, typename R, int T, bool INV>
+__global__ __launch_bounds__(T *(N / Sched<N>::E)) void k_ypass(cx<R> *__restrict__ data, int nfields, Geo g,
+                                                                 const cx<R> *__restrict__ tw) {
+  constexpr int E = Sched<N>::E, TPP = N / E;
+  __shared__ cx<R> lds[T * padlen<N>()];
+  const int p = threadIdx.x % T, t = threadIdx.x / T;
+  const int k = blockIdx.x * T + p;
+  const bool ok = k < N / 2 + 1;
+  cx<R> *base = data + (long)blockIdx.y * N * g.nfp + k;
+  for (int f = 0; f < nfields; f++) {
+    cx<R> *d = base + f * g.fs;
+    cx<R> v[E];
+#pragma unroll
+    for (int c = 0; c < E; c++) v[c] = ok ? d[(long)(t + TPP * c) * g.nfp] : zero<R>();
+    fft_pencil<N, R, INV>(v, t, lds, TileSlot<N, T>{p}, tw);
+    if (ok) {
+#pragma unroll
+      for (int c = 0; c < E; c++) d[(long)(t + TPP * c) * g.nfp] = v[c];
+    }
+  }
+}
+
+// Plain FFT along x of `nfields` fields: in (precision RI) -> out (R).
+// Used for the initial transform and for snapshots; in may equal out.
+// Grid (ceil(nf/T), n): blockIdx.y = j.
+template <int N, typename R, int T, bool INV, typename RI>
+__global__ __launch_bounds__(T *(N / Sched<N>::E)) void k_xplain(const cx<RI> *in, cx<R> *out, int nfields, Geo g,
+                                                                  const cx<R> *__restrict__ tw) {
+  constexpr int E = Sched<N>::E, TPP = N / E;
+  __shared__ cx<R> lds[T * padlen<N>()];
+  const int p = threadIdx.x % T, t = threadIdx.x / T;
+  const int k = blockIdx.x * T + p;
+  const bool ok = k < N / 2 + 1;
+  const long off = (long)blockIdx.y * g.nfp + k, istr = (long)N * g.nfp;
+  for (int f = 0; f < nfields; f++) {
+    cx<R> v[E];
+#pragma unroll
+    for (int c = 0; c < E; c++)
+      v[c] = ok ? cvt<R, RI>(in[f * g.fs + off + (t + TPP * c) * istr]) : zero<R>();
+    fft_pencil<N, R, INV>(v, t, lds, TileSlot<N, T>{p}, tw);
+    if (ok) {
+#pragma unroll
+      for (int c = 0; c < E; c++) out[f * g.fs + off + (t + TPP * c) * istr] = v[c];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// A: curl prologue + inverse FFT along x.
+// Writes omega partials to W fields 3,4,5 and, if WITH_U, the velocity
+// partials to W fields 0,1,2.  The curl follows main.c:359-361:
+//   curlZ = I*(kx V - ky U), curlY = I*(kz U - kx W), curlX = I*(ky W - kz V)
+// with I*z = (-Im z, Re z).  Grid (ceil(nf/T), n): blockIdx.y = j.
+template <int N, typename R, int T, bool WITH_U, typename SI>
+__global__ __launch_bounds__(T *(N / Sched<N>::E)) void k_xinv_curl(const cx<SI> *__restrict__ uh,
+                                                                     cx<R> *__restrict__ W, Geo g,
+                                                                     const cx<R> *__restrict__ tw) {
+  constexpr int E = Sched<N>::E, TPP = N / E;
+  __shared__ cx<R> lds[T * padlen<N>()];
+  const int p = threadIdx.x % T, t = threadIdx.x / T;
+  const int j = blockIdx.y, k = blockIdx.x * T + p;
+  const bool ok = k < N / 2 + 1;
+  const long off = (long)j * g.nfp + k, istr = (long)N * g.nfp;
+  const R ky = (R)wave<N>(j), kz = (R)k;
+  const TileSlot<N, T> slot{p};
+  cx<R> U[E], V[E], Wv[E];
+#pragma unroll
+  for (int c = 0; c < E; c++) {
+    const long l = off + (t + TPP * c) * istr;
+    U[c] = ok ? cvt<R, SI>(uh[l]) : zero<R>();
+    V[c] = ok ? cvt<R, SI>(uh[g.fs + l]) : zero<R>();
+    Wv[c] = ok ? cvt<R, SI>(uh[2 * g.fs + l]) : zero<R>();
+  }
+  auto store = [&](int f, cx<R>(&v)[E]) {
+    if (ok) {
+#pragma unroll
+      for (int c = 0; c < E; c++) W[f * g.fs + off + (t + TPP * c) * istr] = v[c];
+    }
+  };
+  {
+    cx<R> v[E];
+#pragma unroll
+    for (int c = 0; c < E; c++) {  // curlX = I*(ky W - kz V)
+      cx<R> z = mk<R>(ky * Wv[c].x - kz * V[c].x, ky * Wv[c].y - kz * V[c].y);
+      v[c] = mk<R>(-z.y, z.x);
+    }
+    fft_pencil<N, R, true>(v, t, lds, slot, tw);
+    store(3, v);
+  }
+  {
+    cx<R> v[E];
+#pragma unroll
+    for (int c = 0; c < E; c++) {  // curlY = I*(kz U - kx W)
+      const R kx = (R)wave<N>(t + TPP * c);
+      cx<R> z = mk<R>(kz * U[c].x - kx * Wv[c].x, kz * U[c].y - kx * Wv[c].y);
+      v[c] = mk<R>(-z.y, z.x);
+    }
+    fft_pencil<N, R, true>(v, t, lds, slot, tw);
+    store(4, v);
+  }
+  {
+    cx<R> v[E];
+#pragma unroll
+    for (int c = 0; c < E; c++) {  // curlZ = I*(kx V - ky U)
+      const R kx = (R)wave<N>(t + TPP * c);
+      cx<R> z = mk<R>(kx * V[c].x - ky * U[c].x, kx * V[c].y - ky * U[c].y);
+      v[c] = mk<R>(-z.y, z.x);
+    }
+    fft_pencil<N, R, true>(v, t, lds, slot, tw);
+    store(5, v);
+  }
+  if constexpr (WITH_U) {
+    fft_pencil<N, R, true>(U, t, lds, slot, tw);
+    store(0, U);
+    fft_pencil<N, R, true>(V, t, lds, slot, tw);
+    store(1, V);
+    fft_pencil<N, R, true>(Wv, t, lds, slot, tw);
+    store(2, Wv);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// z-pencil helpers.  A block owns PPW contiguous rows (i,j); thread t of a
+// row holds positions pos = t + TPP*c.
+
+// Complex pencil Z = X + iY of two half-spectrum rows extended to length N
+// by Hermitian symmetry; the imaginary parts of k=0 and k=N/2 are dropped
+// (FFTW c2r semantics).  Y == nullptr means Y = 0.
+template <int N, typename R>
+__device__ __forceinline__ void herm_pair(cx<R> (&z)[Sched<N>::E], const cx<R> *X, const cx<R> *Y, int t) {
+  constexpr int E = Sched<N>::E, TPP = N / E, H = N / 2;
+#pragma unroll
+  for (int c = 0; c < E; c++) {
+    const int pos = t + TPP * c;
+    const int k = pos <= H ? pos : N - pos;
+    const cx<R> x = X[k];
+    const cx<R> y = Y ? Y[k] : zero<R>();
+    if (pos == 0 || pos == H) z[c] = mk<R>(x.x, y.x);
+    else if (pos < H) z[c] = mk<R>(x.x - y.y, x.y + y.x);
+    else z[c] = mk<R>(x.x + y.y, y.x - x.y);
+  }
+}
+
+// Split the forward FFT F of (a + i b), a and b real, into the half spectra
+// A_k = (F_k + conj F_{N-k})/2 and B_k = -i (F_k - conj F_{N-k})/2, k <= N/2.
+template <int N, typename R, class Slot>
+__device__ __forceinline__ void split_store(const cx<R> (&f)[Sched<N>::E], cx<R> *A, cx<R> *B, int t, cx<R> *lds,
+                                            Slot slot) {
+  constexpr int E = Sched<N>::E, TPP = N / E, H = N / 2;
+#pragma unroll
+  for (int c = 0; c < E; c++) lds[slot(t + TPP * c)] = f[c];
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c <= E / 2; c++) {
+    const int pos = t + TPP * c;
+    if (pos <= H) {
+      const cx<R> a = lds[slot(pos)];
+      const cx<R> b = lds[slot(pos == 0 ? 0 : N - pos)];
+      const R sx = (R)0.5 * (a.x + b.x), sy = (R)0.5 * (a.y - b.y);
+      const R dx = (R)0.5 * (a.x - b.x), dy = (R)0.5 * (a.y + b.y);
+      A[pos] = mk<R>(sx, sy);
+      B[pos] = mk<R>(dy, -dx);
+    }
+  }
+  __syncthreads();
+}
+
+// C: fused z pass of one RK stage.
+//  MODE 0 (stage 1): u,v,w from the stale physical velocity `us`,
+//                    omega from W fields 3..5;
+//  MODE 1 (stages 2,3): u,v,w,omega from W fields 0..5;
+//  MODE 2 (stage 4): as MODE 1 and the physical u,v,w are written to `us`.
+// Output: N = u x omega (main.c:374-376) transformed along z into W 0..2.
+// Grid n*n/PPW blocks.
+template <int N, typename R, int PPW, int MODE, typename UR>
+__global__ __launch_bounds__(PPW *(N / Sched<N>::E)) void k_zfused(cx<R> *__restrict__ W, UR *__restrict__ us,
+                                                                    Geo g, R invn3,
+                                                                    const cx<R> *__restrict__ tw) {
+  constexpr int E = Sched<N>::E, TPP = N / E, H = N / 2;
+  __shared__ cx<R> lds[PPW * padlen<N>()];
+  const int p = threadIdx.x / TPP, t = threadIdx.x % TPP;
+  const long row = (long)blockIdx.x * PPW + p;
+  cx<R> *w = W + row * g.nfp;
+  const PencilSlot<N> slot{p * padlen<N>()};
+  R u[E], v[E], ww[E], ox[E], oy[E], oz[E];
+  if constexpr (MODE == 0) {
+    cx<R> z1[E], z2[E];
+#pragma unroll
+    for (int c = 0; c < E; c++) {
+      const long l = row * N + t + TPP * c;
+      u[c] = (R)us[l];
+      v[c] = (R)us[g.rs + l];
+      ww[c] = (R)us[2 * g.rs + l];
+    }
+    herm_pair<N, R>(z1, w + 3 * g.fs, w + 4 * g.fs, t);
+    herm_pair<N, R>(z2, w + 5 * g.fs, nullptr, t);
+    fft_pencil<N, R, true>(z1, t, lds, slot, tw);
+    fft_pencil<N, R, true>(z2, t, lds, slot, tw);
+#pragma unroll
+    for (int c = 0; c < E; c++) {
+      ox[c] = z1[c].x * invn3;
+      oy[c] = z1[c].y * invn3;
+      oz[c] = z2[c].x * invn3;
+    }
+  } else {
+    cx<R> z1[E], z2[E], z3[E];
+    herm_pair<N, R>(z1, w, w + g.fs, t);
+    herm_pair<N, R>(z2, w + 2 * g.fs, w + 3 * g.fs, t);
+    herm_pair<N, R>(z3, w + 4 * g.fs, w + 5 * g.fs, t);
+    fft_pencil<N, R, true>(z1, t, lds, slot, tw);
+    fft_pencil<N, R, true>(z2, t, lds, slot, tw);
+    fft_pencil<N, R, true>(z3, t, lds, slot, tw);
+#pragma unroll
+    for (int c = 0; c < E; c++) {
+      u[c] = z1[c].x * invn3;
+      v[c] = z1[c].y * invn3;
+      ww[c] = z2[c].x * invn3;
+      ox[c] = z2[c].y * invn3;
+      oy[c] = z3[c].x * invn3;
+      oz[c] = z3[c].y * invn3;
+    }
+    if constexpr (MODE == 2) {
+#pragma unroll
+      for (int c = 0; c < E; c++) {
+        const long l = row * N + t + TPP * c;
+        us[l] = (UR)u[c];
+        us[g.rs + l] = (UR)v[c];
+        us[2 * g.rs + l] = (UR)ww[c];
+      }
+    }
+  }
+  cx<R> f1[E], f2[E];
+#pragma unroll
+  for (int c = 0; c < E; c++) {
+    f1[c] = mk<R>(v[c] * oz[c] - ww[c] * oy[c], ww[c] * ox[c] - u[c] * oz[c]);
+    f2[c] = mk<R>(u[c] * oy[c] - v[c] * ox[c], (R)0);
+  }
+  fft_pencil<N, R, false>(f1, t, lds, slot, tw);
+  fft_pencil<N, R, false>(f2, t, lds, slot, tw);
+  // f2 is the transform of a real pencil: its first half is the r2c output.
+#pragma unroll
+  for (int c = 0; c <= E / 2; c++) {
+    const int pos = t + TPP * c;
+    if (pos <= H) w[2 * g.fs + pos] = f2[c];
+  }
+  split_store<N, R>(f1, w, w + g.fs, t, lds, slot);
+}
+
+// Plain r2c along z of `nfields` physical double fields into spectral
+// fields of precision R (pairs share one complex transform).
+template <int N, typename R, int PPW>
+__global__ __launch_bounds__(PPW *(N / Sched<N>::E)) void k_zr2c(const double *__restrict__ in, cx<R> *out,
+                                                                  int nfields, Geo g,
+                                                                  const cx<R> *__restrict__ tw) {
+  constexpr int E = Sched<N>::E, TPP = N / E, H = N / 2;
+  __shared__ cx<R> lds[PPW * padlen<N>()];
+  const int p = threadIdx.x / TPP, t = threadIdx.x % TPP;
+  const long row = (long)blockIdx.x * PPW + p;
+  const PencilSlot<N> slot{p * padlen<N>()};
+  for (int f = 0; f < nfields; f += 2) {
+    const bool pair = f + 1 < nfields;
+    cx<R> z[E];
+#pragma unroll
+    for (int c = 0; c < E; c++) {
+      const long l = row * N + t + TPP * c;
+      z[c] = mk<R>((R)in[f * g.rs + l], pair ? (R)in[(f + 1) * g.rs + l] : (R)0);
+    }
+    fft_pencil<N, R, false>(z, t, lds, slot, tw);
+    cx<R> *o = out + row * g.nfp + f * g.fs;
+    if (pair) {
+      split_store<N, R>(z, o, o + g.fs, t, lds, slot);
+    } else {
+#pragma unroll
+      for (int c = 0; c <= E / 2; c++) {
+        const int pos = t + TPP * c;
+        if (pos <= H) o[pos] = z[c];
+      }
+    }
+  }
+}
+
+// Plain c2r along z of one spectral field into a physical double field,
+// multiplied by `scale` (1 for the raw transform, 1/n^3 for snapshots).
+template <int N, typename R, int PPW>
+__global__ __launch_bounds__(PPW *(N / Sched<N>::E)) void k_zc2r(const cx<R> *__restrict__ in,
+                                                                  double *__restrict__ out, Geo g, R scale,
+                                                                  const cx<R> *__restrict__ tw) {
+  constexpr int E = Sched<N>::E, TPP = N / E;
+  __shared__ cx<R> lds[PPW * padlen<N>()];
+  const int p = threadIdx.x / TPP, t = threadIdx.x % TPP;
+  const long row = (long)blockIdx.x * PPW + p;
+  const PencilSlot<N> slot{p * padlen<N>()};
+  cx<R> z[E];
+  herm_pair<N, R>(z, in + row * g.nfp, nullptr, t);
+  fft_pencil<N, R, true>(z, t, lds, slot, tw);
+#pragma unroll
+  for (int c = 0; c < E; c++) out[row * N + t + TPP * c] = (double)(z[c].x * scale);
+}
+
+// ---------------------------------------------------------------------------
+// E: forward FFT along x of the three N partials + spectral epilogue.
+//  dU = N*(dealias*dt); P = kk>0 ? (dU kx + dV ky + dW kz)/kk : 0;
+//  dU -= P kx + (nu dt kk) Ur   (main.c:381-399, same evaluation order)
+//  STAGE 0  : Ur = Un;  Ur' = Un + b dU;  acc = Un + a dU
+//  STAGE 1,2: Ur' = Un + b dU;  acc += a dU
+//  STAGE 3  : Un = acc + a dU                     (main.c:400-420)
+// The epilogue runs in double whatever R is.  phat (optional) receives P.
+// Grid (ceil(nf/T), n): blockIdx.y = j.
+template <int N, typename R, int T, int STAGE, typename SR>
+__global__ __launch_bounds__(T *(N / Sched<N>::E)) void k_xfwd_rk(const cx<R> *__restrict__ W,
+                                                                   cx<SR> *__restrict__ uhr,
+                                                                   double2 *__restrict__ uhn,
+                                                                   double2 *__restrict__ acc,
+                                                                   double2 *__restrict__ phat, Geo g, RKParams q,
+                                                                   const cx<R> *__restrict__ tw) {
+#pragma clang fp contract(off)
+  constexpr int E = Sched<N>::E, TPP = N / E;
+  __shared__ cx<R> lds[T * padlen<N>()];
+  const int p = threadIdx.x % T, t = threadIdx.x / T;
+  const int j = blockIdx.y, k = blockIdx.x * T + p;
+  const bool ok = k < N / 2 + 1;
+  const long off = (long)j * g.nfp + k, istr = (long)N * g.nfp;
+  const TileSlot<N, T> slot{p};
+  cx<R> F[3][E];
+#pragma unroll
+  for (int f = 0; f < 3; f++) {
+#pragma unroll
+    for (int c = 0; c < E; c++) F[f][c] = ok ? W[f * g.fs + off + (t + TPP * c) * istr] : zero<R>();
+  }
+#pragma unroll
+  for (int f = 0; f < 3; f++) fft_pencil<N, R, false>(F[f], t, lds, slot, tw);
+  if (!ok) return;
+  const double ky = (double)wave<N>(j), kz = (double)k;
+  const double nudt = q.nu * q.dt;
+  const bool myz = fabs(ky) < q.kmax && fabs(kz) < q.kmax;
+#pragma unroll
+  for (int c = 0; c < E; c++) {
+    const long l = off + (t + TPP * c) * istr;
+    const double kx = (double)wave<N>(t + TPP * c);
+    const double kk = kx * kx + ky * ky + kz * kz;
+    const double fac = (myz && fabs(kx) < q.kmax) ? q.dt : 0.0;
+    double2 d[3];
+#pragma unroll
+    for (int f = 0; f < 3; f++) d[f] = make_double2((double)F[f][c].x * fac, (double)F[f][c].y * fac);
+    double2 P = make_double2(0.0, 0.0);
+    if (kk > 0) {
+      P.x = (d[0].x * kx + d[1].x * ky + d[2].x * kz) / kk;
+      P.y = (d[0].y * kx + d[1].y * ky + d[2].y * kz) / kk;
+    }
+    if (phat) phat[l] = P;
+    const double vk = nudt * kk;
+    const double kv[3] = {kx, ky, kz};
+#pragma unroll
+    for (int f = 0; f < 3; f++) {
+      const long lf = f * g.fs + l;
+      double2 ur;
+      if constexpr (STAGE == 0) ur = uhn[lf];
+      else ur = cvt<double, SR>(uhr[lf]);
+      d[f].x -= P.x * kv[f] + vk * ur.x;
+      d[f].y -= P.y * kv[f] + vk * ur.y;
+      if constexpr (STAGE == 0) {
+        uhr[lf] = cvt<SR, double>(make_double2(ur.x + q.b * d[f].x, ur.y + q.b * d[f].y));
+        acc[lf] = make_double2(ur.x + q.a * d[f].x, ur.y + q.a * d[f].y);
+      } else if constexpr (STAGE < 3) {
+        const double2 un = uhn[lf];
+        uhr[lf] = cvt<SR, double>(make_double2(un.x + q.b * d[f].x, un.y + q.b * d[f].y));
+        const double2 ac = acc[lf];
+        acc[lf] = make_double2(ac.x + q.a * d[f].x, ac.y + q.a * d[f].y);
+      } else {
+        const double2 ac = acc[lf];
+        uhn[lf] = make_double2(ac.x + q.a * d[f].x, ac.y + q.a * d[f].y);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Energy / enstrophy (main.c:257-266): per element
+//   e = |U|^2 + |V|^2 + |W|^2 (double, left to right), z = kk * e,
+// summed with compensated (two-sum) accumulation per thread, then a
+// compensated tree per block; partial sums go to part[4*block].
+__device__ __forceinline__ void two_sum_acc(double &s, double &c, double x) {
+  const double t = s + x;
+  const double bp = t - s;
+  c += (s - (t - bp)) + (x - bp);
+  s = t;
+}
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_diag(const double2 *__restrict__ uh, int n, Geo g,
+                                             double *__restrict__ part) {
+#pragma clang fp contract(off)
+  const int nf = n / 2 + 1;
+  double es = 0, ec = 0, zs = 0, zc = 0;
+  for (long row = blockIdx.x; row < (long)n * n; row += gridDim.x) {
+    const int i = (int)(row / n), j = (int)(row % n);
+    const double kx = i < n / 2 ? i : i - n, ky = j < n / 2 ? j : j - n;
+    const double2 *r = uh + row * g.nfp;
+    for (int k = threadIdx.x; k < nf; k += BS) {
+      const double2 a = r[k], b = r[g.fs + k], c = r[2 * g.fs + k];
+      const double e = (a.x * a.x + a.y * a.y) + (b.x * b.x + b.y * b.y) + (c.x * c.x + c.y * c.y);
+      const double kk = kx * kx + ky * ky + (double)k * (double)k;
+      two_sum_acc(es, ec, e);
+      two_sum_acc(zs, zc, kk * e);
+    }
+  }
+  __shared__ double sh[4][BS];
+  sh[0][threadIdx.x] = es;
+  sh[1][threadIdx.x] = ec;
+  sh[2][threadIdx.x] = zs;
+  sh[3][threadIdx.x] = zc;
+  __syncthreads();
+  for (int s = BS / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      double a = sh[0][threadIdx.x], ac = sh[1][threadIdx.x];
+      two_sum_acc(a, ac, sh[0][threadIdx.x + s]);
+      sh[0][threadIdx.x] = a;
+      sh[1][threadIdx.x] = ac + sh[1][threadIdx.x + s];
+      double z = sh[2][threadIdx.x], zcc = sh[3][threadIdx.x];
+      two_sum_acc(z, zcc, sh[2][threadIdx.x + s]);
+      sh[2][threadIdx.x] = z;
+      sh[3][threadIdx.x] = zcc + sh[3][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) part[4 * blockIdx.x + q] = sh[q][0];
+  }
+}
+
+// Final compensated reduction of `nb` block partials by one block.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_diag_final(const double *__restrict__ part, int nb,
+                                                   double *__restrict__ out) {
+#pragma clang fp contract(off)
+  double es = 0, ec = 0, zs = 0, zc = 0;
+  for (int b = threadIdx.x; b < nb; b += BS) {
+    two_sum_acc(es, ec, part[4 * b]);
+    ec += part[4 * b + 1];
+    two_sum_acc(zs, zc, part[4 * b + 2]);
+    zc += part[4 * b + 3];
+  }
+  __shared__ double sh[4][BS];
+  sh[0][threadIdx.x] = es;
+  sh[1][threadIdx.x] = ec;
+  sh[2][threadIdx.x] = zs;
+  sh[3][threadIdx.x] = zc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0, ac = 0, z = 0, zcc = 0;
+    for (int q = 0; q < BS; q++) {
+      two_sum_acc(a, ac, sh[0][q]);
+      ac += sh[1][q];
+      two_sum_acc(z, zcc, sh[2][q]);
+      zcc += sh[3][q];
+    }
+    out[0] = a;
+    out[1] = ac;
+    out[2] = z;
+    out[3] = zcc;
+  }
+}
+
+}  // namespace dnsk

@@ -1,0 +1,77 @@
+"""Algorithmic HBM bytes of the step's kernels (the roofline numerator).
+
+Counting rule: every logical array element a kernel must read or write is
+counted once, at the precision it is stored in; padding, cache re-reads and
+twiddle tables are not counted.  With Uc = n^2 (n/2+1) complex elements per
+spectral field and Rr = n^3 reals per physical field:
+
+  pass  stage 1                 stages 2,3                stage 4
+  A     r 3Uc(f64) w 3Uc        r 3Uc(SR) w 6Uc           as 2,3
+  B     r 3Uc w 3Uc             r 6Uc w 6Uc               as 2,3
+  C     r 3Uc+3Rr w 3Uc         r 6Uc w 3Uc               r 6Uc w 3Uc+3Rr
+  D     r 3Uc w 3Uc             as 1                      as 1
+  E     r 3Uc+3Uc(f64)          r 3Uc+3Uc(SR)+6Uc(f64)    r 3Uc+3Uc(SR)+3Uc(f64)
+        w 3Uc(SR)+3Uc(f64)      w 3Uc(SR)+3Uc(f64)        w 3Uc(f64)
+
+Partials (A..E outputs) and the stale velocity are in the working precision
+R; the step-start state and the accumulator are always double; SR is the
+stage state's precision (= R).  FP64 total: 198 field-units per step
+(a unit = one spectral or physical field at 8 bytes per real scalar).
+"""
+
+PEAK_HBM_GBS = 8000.0  # MI355X HBM3E, published peak
+PASSES = ("xinv_curl", "ypass_inv", "zfused", "ypass_fwd", "xfwd_rk")
+
+
+def slot_bytes(n, precision="fp64"):
+    """{slot name: algorithmic bytes of one launch}; slot = pass.s<stage>."""
+    uc = n * n * (n // 2 + 1)
+    rr = n ** 3
+    r = 8 if precision == "fp64" else 4      # real scalar in the working precision
+    cr, csr, c64 = 2 * r, 2 * r, 16
+    out = {}
+    for st in range(4):
+        a = 3 * uc * c64 + 3 * uc * cr if st == 0 else 3 * uc * csr + 6 * uc * cr
+        b = (2 * 3 if st == 0 else 2 * 6) * uc * cr
+        if st == 0:
+            c = 3 * uc * cr + 3 * rr * r + 3 * uc * cr
+        elif st < 3:
+            c = 6 * uc * cr + 3 * uc * cr
+        else:
+            c = 6 * uc * cr + 3 * uc * cr + 3 * rr * r
+        d = 2 * 3 * uc * cr
+        if st == 0:
+            e = 3 * uc * cr + 3 * uc * c64 + 3 * uc * csr + 3 * uc * c64
+        elif st < 3:
+            e = 3 * uc * cr + 3 * uc * csr + 6 * uc * c64 + 3 * uc * csr + 3 * uc * c64
+        else:
+            e = 3 * uc * cr + 3 * uc * csr + 3 * uc * c64 + 3 * uc * c64
+        for p, v in zip(PASSES, (a, b, c, d, e)):
+            out[f"{p}.s{st + 1}"] = v
+    return out
+
+
+def step_bytes(n, precision="fp64"):
+    return sum(slot_bytes(n, precision).values())
+
+
+def summarize(timing, n, precision, steps):
+    """Per pass: launches, average ms, algorithmic GB/s and fraction of peak,
+    from {slot: (total ms, launches)} accumulated over `steps` steps."""
+    sb = slot_bytes(n, precision)
+    per_pass = {}
+    for slot, (ms, cnt) in timing.items():
+        if cnt == 0:
+            continue
+        p = slot.split(".")[0]
+        d = per_pass.setdefault(p, {"ms": 0.0, "launches": 0, "bytes": 0})
+        d["ms"] += ms
+        d["launches"] += cnt
+        d["bytes"] += sb[slot] * cnt
+    for p, d in per_pass.items():
+        d["avg_ms"] = d["ms"] / d["launches"]
+        d["GBs"] = d["bytes"] / (d["ms"] * 1e-3) / 1e9
+        d["frac"] = d["GBs"] / PEAK_HBM_GBS
+        d["ms_per_step"] = d["ms"] / steps
+        d["bytes_per_launch"] = d["bytes"] / d["launches"]
+    return per_pass

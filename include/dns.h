@@ -1,0 +1,125 @@
+/*
+ * libdns_hip: MI355X (gfx950) implementation of the RK4 time step of the
+ * pseudo-spectral Navier-Stokes solver `dns` (slitvinov/dns, main.c) for
+ * the triply periodic box [0, 2 pi)^3.
+ *
+ * C ABI: plain pointers and sizes.  Every call returns DNS_OK (0) or a
+ * negative DNS_E* code; the library never exits the process.  One context
+ * owns all device memory and one HIP stream; calls on a context must be
+ * serialised, separate contexts may be used from separate host threads.
+ * Host buffers are owned by the caller and only read or written during
+ * the call.
+ *
+ * Which reference interface each entry point replaces (main.c of the
+ * reference, which has no library boundary; the boundary it crosses for
+ * this path is FFTW's plan/execute API):
+ *   dns_create          main.c:174-251  sizing, allocation, fftw_plan_dft_*_3d
+ *                                       (221-223), kx/kz/kmax/dealias/kk
+ *                                       tables (224-247)
+ *   dns_load_velocity   main.c:180-185,249-251  read U,V,W and the initial
+ *                                       fftw_execute_dft_r2c of each
+ *   dns_step            main.c:334-422  one RK4 step per call unit (33
+ *                                       fftw_execute_dft_{r2c,c2r} + the
+ *                                       OpenMP loops of the stage)
+ *   dns_diagnostics     main.c:257-266  energy / enstrophy reduction
+ *   dns_snapshot        main.c:269-280  c2r of U,V,W,P scaled by 1/n^3
+ *   dns_rfft3/irfft3    main.c:221-223  fftw_plan_dft_r2c_3d / c2r_3d
+ *                                       executed on caller data
+ *   dns_destroy         main.c:424-460  fftw_destroy_plan / fftw_free
+ */
+#ifndef DNS_H
+#define DNS_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  DNS_OK = 0,
+  DNS_EINVAL = -1,      /* bad argument (grid size, null pointer, ...) */
+  DNS_ENODEV = -2,      /* no usable GPU */
+  DNS_ENOMEM = -3,      /* device allocation failed */
+  DNS_EHIP = -4,        /* other HIP runtime error */
+  DNS_ENONFINITE = -5   /* energy or enstrophy is NaN or Inf */
+};
+
+/* Arithmetic of the step.
+ * DNS_FP64 : everything in double (the reference's arithmetic).
+ * DNS_MIXED: FFTs, curl and u x omega in float with twiddles generated in
+ *            double; energy/enstrophy reductions, the RK4 accumulator and
+ *            the step-start spectral state in double. */
+enum dns_precision { DNS_FP64 = 0, DNS_MIXED = 1 };
+
+typedef struct dns_ctx dns_ctx;
+
+/* Grid n x n x n with n = 2^a 3^b, 16 <= n <= 1024 (n=16,32,48,64,96,128,
+ * 192,256,384,512,768,1024).  nu = viscosity (-n), dt = time step (-s).
+ * device = HIP device ordinal.  On success *out holds a new context. */
+int dns_create(dns_ctx **out, int n, double nu, double dt, int precision, int device);
+void dns_destroy(dns_ctx *ctx);
+
+/* Grid size of the context. */
+int dns_get_n(const dns_ctx *ctx);
+
+/* uvw: host [3][n][n][n] doubles (U, V, W of the raw input file).  Sets the
+ * spectral state to the r2c transform of u (no projection, no dealiasing,
+ * as main.c:249-251) and the stale physical velocity used by the first
+ * stage of the next step to u itself (main.c:180-185). */
+int dns_load_velocity(dns_ctx *ctx, const double *uvw);
+
+/* Advance nsteps RK4 steps (asynchronous on the context's stream). */
+int dns_step(dns_ctx *ctx, long nsteps);
+
+/* Ask the last step of every following dns_step call to keep the stage-4
+ * pressure P = (k . dU)/|k|^2 for dns_snapshot (main.c:392-395).  Off by
+ * default: it costs one extra field write per call. */
+int dns_set_pressure_capture(dns_ctx *ctx, int on);
+
+/* Energy sum |U|^2+|V|^2+|W|^2 / n^6 and enstrophy sum |k|^2(...) / n^6
+ * over the half spectrum (main.c:257-266), accumulated in double with
+ * compensated summation.  Synchronises the stream.  Returns
+ * DNS_ENONFINITE (after storing the values) if either is NaN or Inf. */
+int dns_diagnostics(dns_ctx *ctx, double *energy, double *enstrophy);
+
+/* Host [4][n][n][n] doubles: c2r(U,V,W,P)/n^3 (main.c:269-280).  P is the
+ * captured stage-4 pressure, or zeros if none was captured. */
+int dns_snapshot(dns_ctx *ctx, double *uvwp);
+
+/* State access in the reference layouts, for checkpointing and tests.
+ * Spectral: host [3][n][n][n/2+1] complex (interleaved re,im doubles).
+ * Stale physical velocity: host [3][n][n][n] doubles. */
+int dns_get_spectral(dns_ctx *ctx, double *uvw_hat);
+int dns_set_spectral(dns_ctx *ctx, const double *uvw_hat);
+int dns_get_stale_velocity(dns_ctx *ctx, double *uvw);
+int dns_set_stale_velocity(dns_ctx *ctx, const double *uvw);
+
+/* Stand-alone 3-D transforms on host data with the context's kernels and
+ * precision, FFTW conventions (unnormalised; r2c forward exp(-i), c2r
+ * backward exp(+i) ignoring Im of the kz=0 and kz=n/2 coefficients).
+ * real: [n][n][n] doubles; spec: [n][n][n/2+1] complex doubles. */
+int dns_rfft3(dns_ctx *ctx, const double *real, double *spec);
+int dns_irfft3(dns_ctx *ctx, const double *spec, double *real);
+
+/* Kernel timing.  When on, every kernel launched by dns_step is bracketed
+ * by HIP events on the context's stream; times accumulate per slot
+ * 5*stage + pass (stage 0..3, pass 0..4 = x-inverse with curl, y-inverse,
+ * fused z pass, y-forward, x-forward with the RK update).  Switching on or
+ * off resets the sums.  dns_kernel_timing synchronises and fills
+ * ms[DNS_KERNEL_SLOTS] and launches[DNS_KERNEL_SLOTS]. */
+#define DNS_KERNEL_SLOTS 20
+int dns_set_kernel_timing(dns_ctx *ctx, int on);
+int dns_kernel_timing(dns_ctx *ctx, double *ms, long *launches);
+const char *dns_kernel_slot_name(int slot);
+
+/* Bytes of device memory held by the context. */
+long long dns_device_bytes(const dns_ctx *ctx);
+
+/* Static text for an error code. */
+const char *dns_strerror(int code);
+
+/* Detail of the last error on this thread (HIP message), or "". */
+const char *dns_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,191 @@
+"""Shared fixtures: the CPU oracle (ctypes over oracle/_build/liboracle.so),
+Taylor-Green and random input fields, the golden trajectories."""
+import ctypes
+import json
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_LIB = os.path.join(ORACLE_DIR, "_build", "liboracle.so")
+ORACLE_CLI = os.path.join(ORACLE_DIR, "_build", "dns_cpu")
+DNS_CLI = os.path.join(ROOT, "dns_amd", "bin", "dns")
+DNS_IC = os.path.join(ROOT, "dns_amd", "bin", "dns-ic")
+GOLDEN = os.path.join(ROOT, "tests", "golden", "tgv_trajectories.json")
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an AMD GPU (MI355X) and the built HIP library")
+
+
+def _build_oracle():
+    if not (os.path.exists(ORACLE_LIB) and os.path.exists(ORACLE_CLI)):
+        subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+
+
+class Oracle:
+    """ctypes view of oracle/oracle.h."""
+
+    def __init__(self):
+        _build_oracle()
+        L = ctypes.CDLL(ORACLE_LIB)
+        P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+        L.orc_fft1.argtypes = [I, P, ctypes.c_long, I]
+        L.orc_rfft3.argtypes = [I, P, P]
+        L.orc_irfft3.argtypes = [I, P, P]
+        L.orc_create.restype = P
+        L.orc_create.argtypes = [I, D, D]
+        L.orc_destroy.argtypes = [P]
+        L.orc_load.argtypes = [P, P]
+        L.orc_step.argtypes = [P, ctypes.c_long]
+        L.orc_diag.argtypes = [P, ctypes.POINTER(ctypes.c_longdouble), ctypes.POINTER(ctypes.c_longdouble)]
+        L.orc_diag_d.argtypes = [P, ctypes.POINTER(D), ctypes.POINTER(D)]
+        for f in ("orc_get_spectral", "orc_set_spectral", "orc_get_ustale", "orc_set_ustale",
+                  "orc_get_phat", "orc_snapshot"):
+            getattr(L, f).argtypes = [P, P]
+        self.L = L
+
+    def rfft3(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        n = a.shape[0]
+        out = np.empty((n, n, n // 2 + 1), dtype=np.complex128)
+        assert self.L.orc_rfft3(n, a.ctypes.data, out.ctypes.data) == 0
+        return out
+
+    def irfft3(self, s):
+        s = np.ascontiguousarray(s, dtype=np.complex128)
+        n = s.shape[0]
+        out = np.empty((n, n, n), dtype=np.float64)
+        assert self.L.orc_irfft3(n, s.ctypes.data, out.ctypes.data) == 0
+        return out
+
+    def fft1(self, a, sign):
+        a = np.ascontiguousarray(a, dtype=np.complex128).copy()
+        assert self.L.orc_fft1(a.shape[-1], a.ctypes.data, a.size // a.shape[-1], sign) == 0
+        return a
+
+    def solver(self, n, nu, dt):
+        return OracleSolver(self, n, nu, dt)
+
+
+class OracleSolver:
+    def __init__(self, orc, n, nu, dt):
+        self.o, self.n = orc, n
+        self.p = orc.L.orc_create(n, nu, dt)
+
+    def close(self):
+        if self.p:
+            self.o.L.orc_destroy(self.p)
+            self.p = None
+
+    def __del__(self):
+        self.close()
+
+    def load_velocity(self, uvw):
+        uvw = np.ascontiguousarray(uvw, dtype=np.float64)
+        self.o.L.orc_load(self.p, uvw.ctypes.data)
+
+    def step(self, k=1):
+        self.o.L.orc_step(self.p, k)
+
+    def diagnostics(self):
+        e, z = ctypes.c_longdouble(), ctypes.c_longdouble()
+        self.o.L.orc_diag(self.p, ctypes.byref(e), ctypes.byref(z))
+        return e.value, z.value
+
+    def get_spectral(self):
+        n = self.n
+        out = np.empty((3, n, n, n // 2 + 1), dtype=np.complex128)
+        self.o.L.orc_get_spectral(self.p, out.ctypes.data)
+        return out
+
+    def set_spectral(self, a):
+        a = np.ascontiguousarray(a, dtype=np.complex128)
+        self.o.L.orc_set_spectral(self.p, a.ctypes.data)
+
+    def get_stale_velocity(self):
+        n = self.n
+        out = np.empty((3, n, n, n), dtype=np.float64)
+        self.o.L.orc_get_ustale(self.p, out.ctypes.data)
+        return out
+
+    def set_stale_velocity(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        self.o.L.orc_set_ustale(self.p, a.ctypes.data)
+
+    def get_phat(self):
+        n = self.n
+        out = np.empty((n, n, n // 2 + 1), dtype=np.complex128)
+        self.o.L.orc_get_phat(self.p, out.ctypes.data)
+        return out
+
+    def snapshot(self):
+        n = self.n
+        out = np.empty((4, n, n, n), dtype=np.float64)
+        self.o.L.orc_snapshot(self.p, out.ctypes.data)
+        return out
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    return Oracle()
+
+
+def tgv_field(n):
+    """Taylor-Green vortex as tgv.py:14-27 of the reference builds it."""
+    dx = 2 * math.pi / n
+    i = np.arange(n)
+    s, c = np.sin(dx * i), np.cos(dx * i)
+    a = np.zeros((4, n, n, n))
+    a[0] = s[:, None, None] * c[None, :, None] * c[None, None, :]
+    a[1] = (-c)[:, None, None] * s[None, :, None] * c[None, None, :]
+    return a
+
+
+def random_field(n, seed, nyquist=True):
+    """Smooth-ish random real velocity (not divergence-free), with content
+    in the Nyquist planes unless nyquist=False."""
+    rng = np.random.default_rng(seed)
+    u = rng.standard_normal((3, n, n, n))
+    uh = np.fft.rfftn(u, axes=(1, 2, 3))
+    k = np.fft.fftfreq(n, 1.0 / n)
+    kz = np.arange(n // 2 + 1)
+    kk = k[:, None, None] ** 2 + k[None, :, None] ** 2 + kz[None, None, :] ** 2
+    uh *= np.exp(-kk / (0.08 * n * n))[None]
+    if not nyquist:
+        uh[:, n // 2, :, :] = 0
+        uh[:, :, n // 2, :] = 0
+        uh[:, :, :, n // 2] = 0
+    return np.fft.irfftn(uh, s=(n, n, n), axes=(1, 2, 3))
+
+
+def write_raw(path, uvw, p=None):
+    n = uvw.shape[-1]
+    a = np.zeros((4, n, n, n))
+    a[:3] = uvw[:3]
+    if p is not None:
+        a[3] = p
+    a.tofile(path)
+
+
+def load_golden():
+    with open(GOLDEN) as f:
+        return json.load(f)
+
+
+def parse_lines(text):
+    rows = []
+    for line in text.splitlines():
+        parts = line.split()
+        if len(parts) == 4 and parts[0].lstrip("-").isdigit():
+            rows.append((int(parts[0]), float(parts[1]), float(parts[2]), float(parts[3])))
+    return rows
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))

@@ -1,0 +1,120 @@
+"""GPU RK4 step (FP64) vs the CPU oracle on identical inputs, vs the
+reference's golden trajectory, and the mixed form's error budget.
+
+Bars: FP64 state and diagnostics agree with the oracle to 1e-12 relative
+(max norm); mixed energy within 1e-5 relative of the FP64 GPU run."""
+import numpy as np
+import pytest
+
+import dns_amd
+from conftest import load_golden, random_field, rel, tgv_field
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(oracle, n, nu, dt, uvw, precision="fp64"):
+    g = dns_amd.Solver(n, nu, dt, precision)
+    o = oracle.solver(n, nu, dt)
+    g.load_velocity(uvw)
+    o.load_velocity(uvw)
+    return g, o
+
+
+@pytest.mark.parametrize("n", [16, 32, 48, 64])
+def test_load_matches_oracle(oracle, n):
+    print(f"load n={n}", flush=True)
+    u = random_field(n, seed=n)
+    g, o = _pair(oracle, n, 0.01, 0.01, u)
+    assert rel(g.get_spectral(), o.get_spectral()) < 1e-14
+    assert rel(g.get_stale_velocity(), o.get_stale_velocity()) == 0.0
+    ge, gz = g.diagnostics()
+    oe, oz = o.diagnostics()
+    assert abs(ge - oe) / oe < 1e-14 and abs(gz - oz) / oz < 1e-14
+    g.close()
+
+
+@pytest.mark.parametrize("n,seed", [(16, 1), (32, 2), (48, 3), (64, 4), (96, 5)])
+def test_steps_match_oracle(oracle, n, seed):
+    print(f"steps n={n}", flush=True)
+    u = random_field(n, seed=seed)
+    nu, dt = 0.02, 0.005
+    g, o = _pair(oracle, n, nu, dt, u)
+    for k in (1, 2, 5):
+        g.step(k)
+        o.step(k)
+        assert rel(g.get_spectral(), o.get_spectral()) < 1e-12, k
+        assert rel(g.get_stale_velocity(), o.get_stale_velocity()) < 1e-12, k
+        ge, gz = g.diagnostics()
+        oe, oz = o.diagnostics()
+        assert abs(ge - oe) / oe < 1e-13 and abs(gz - oz) / oz < 1e-13
+    g.close()
+
+
+def test_tgv_initial_diagnostics():
+    with dns_amd.Solver(64, 0.01, 0.01) as g:
+        g.load_velocity(tgv_field(64)[:3])
+        e, z = g.diagnostics()
+    assert abs(e - 0.125) < 1e-16 and abs(z - 0.375) < 1e-15
+
+
+def test_golden_trajectory_128_re100():
+    gold = load_golden()["128/100"]
+    n, nu, dt = gold["n"], float(gold["nu"]), gold["dt"]
+    rows = {r[0]: (float(r[2]), float(r[3])) for r in gold["rows"]}
+    print("golden 128/100, 100 steps", flush=True)
+    with dns_amd.Solver(n, nu, dt) as g:
+        g.load_velocity(tgv_field(n)[:3])
+        for step in range(0, 101, 10):
+            if step:
+                g.step(10)
+            e, z = g.diagnostics()
+            ge, gz = rows[step]
+            assert abs(e - ge) / ge < 1e-12, (step, e, ge)
+            assert abs(z - gz) / gz < 1e-12, (step, z, gz)
+
+
+def test_pressure_and_snapshot_match_oracle(oracle):
+    n, nu, dt = 32, 0.02, 0.01
+    u = random_field(n, seed=11)
+    g, o = _pair(oracle, n, nu, dt, u)
+    snap0 = g.snapshot()
+    assert np.all(snap0[3] == 0)
+    g.set_pressure_capture(True)
+    g.step(3)
+    o.step(3)
+    so, sg = o.snapshot(), g.snapshot()
+    assert rel(sg[:3], so[:3]) < 1e-12
+    assert rel(sg[3], so[3]) < 1e-11
+    g.close()
+
+
+def test_state_roundtrip_and_restart(oracle):
+    n = 32
+    u = random_field(n, seed=5)
+    g, o = _pair(oracle, n, 0.02, 0.01, u)
+    o.step(2)
+    g.set_spectral(o.get_spectral())
+    g.set_stale_velocity(o.get_stale_velocity())
+    assert rel(g.get_spectral(), o.get_spectral()) == 0.0
+    g.step(2)
+    o.step(2)
+    assert rel(g.get_spectral(), o.get_spectral()) < 1e-12
+    g.close()
+
+
+def test_mixed_energy_error_tgv():
+    n, nu, dt = 64, 0.01, 0.01
+    print("mixed vs fp64 n=64, 100 steps", flush=True)
+    with dns_amd.Solver(n, nu, dt, "fp64") as a, dns_amd.Solver(n, nu, dt, "mixed") as b:
+        u = tgv_field(n)[:3]
+        a.load_velocity(u)
+        b.load_velocity(u)
+        worst = 0.0
+        for _ in range(10):
+            a.step(10)
+            b.step(10)
+            ea, za = a.diagnostics()
+            eb, zb = b.diagnostics()
+            worst = max(worst, abs(eb - ea) / ea)
+        print("mixed max rel energy error", worst)
+        assert worst < 1e-5
