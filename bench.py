@@ -118,7 +118,7 @@ def main():
                  "passes": {k: {"avg_ms": v["avg_ms"], "GBs": v["GBs"], "frac": v["frac"]}
                             for k, v in rmx["passes"].items()}}
     dom_name, dom = max(r64["passes"].items(), key=lambda kv: kv[1]["ms"])
-    step_b = roofline.step_bytes(n, "fp64")
+    step_b = roofline.step_bytes_fused(n, "fp64")
     line = {
         "metric": METRIC, "value": r64["steps_per_s"], "unit": "steps/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": r64["ms_per_step"],

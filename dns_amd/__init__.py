@@ -12,14 +12,14 @@ import os
 
 import numpy as np
 
-__all__ = ["Solver", "DnsError", "lib", "LIB_PATH", "FP64", "MIXED", "abi_functions"]
+__all__ = ["Solver", "Batch", "DnsError", "lib", "LIB_PATH", "FP64", "MIXED", "abi_functions"]
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 LIB_PATH = os.path.join(PKG, "libdns_hip.so")
 HEADER = os.path.join(ROOT, "include", "dns.h")
 FP64, MIXED = 0, 1
-KERNEL_SLOTS = 20
+KERNEL_SLOTS = 24
 
 DNS_OK, DNS_EINVAL, DNS_ENODEV, DNS_ENOMEM, DNS_EHIP, DNS_ENONFINITE = 0, -1, -2, -3, -4, -5
 
@@ -62,6 +62,15 @@ def lib():
         "dns_set_kernel_timing": (I, [P, I]),
         "dns_kernel_timing": (I, [P, D, ctypes.POINTER(LL)]),
         "dns_kernel_slot_name": (ctypes.c_char_p, [I]),
+        "dns_batch_create": (I, [ctypes.POINTER(P), I, P, P, P, I, I]),
+        "dns_batch_destroy": (None, [P]),
+        "dns_batch_load_velocity": (I, [P, I, P]),
+        "dns_batch_step": (I, [P, LL]),
+        "dns_batch_diagnostics": (I, [P, I, D, D]),
+        "dns_batch_get_spectral": (I, [P, I, P]),
+        "dns_batch_get_stale_velocity": (I, [P, I, P]),
+        "dns_batch_member_n": (I, [P, I]),
+        "dns_batch_device_bytes": (ctypes.c_longlong, [P]),
         "dns_strerror": (ctypes.c_char_p, [I]),
         "dns_last_error": (ctypes.c_char_p, []),
     }
@@ -190,4 +199,69 @@ class Solver:
         assert spec.shape == (self.n, self.n, self.nf)
         out = np.empty((self.n,) * 3, dtype=np.float64)
         _check(lib().dns_irfft3(self._ctx, _ptr(spec), _ptr(out)), "dns_irfft3")
+        return out
+
+
+class Batch:
+    """An ensemble of independent simulations (include/dns.h, dns_batch_*)."""
+
+    def __init__(self, ns, nus, dts, precision=FP64, device=0):
+        if isinstance(precision, str):
+            precision = {"fp64": FP64, "mixed": MIXED}[precision]
+        self.ns = [int(v) for v in ns]
+        m = len(self.ns)
+        n_arr = np.ascontiguousarray(self.ns, dtype=np.int32)
+        nu_arr = np.ascontiguousarray(np.broadcast_to(np.asarray(nus, dtype=np.float64), (m,)))
+        dt_arr = np.ascontiguousarray(np.broadcast_to(np.asarray(dts, dtype=np.float64), (m,)))
+        self._b = ctypes.c_void_p()
+        _check(lib().dns_batch_create(ctypes.byref(self._b), m, _ptr(n_arr), _ptr(nu_arr), _ptr(dt_arr),
+                                      precision, device), "dns_batch_create")
+
+    def close(self):
+        if self._b:
+            lib().dns_batch_destroy(self._b)
+            self._b = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __len__(self):
+        return len(self.ns)
+
+    def device_bytes(self):
+        return lib().dns_batch_device_bytes(self._b)
+
+    def load_velocity(self, m, uvw):
+        uvw = np.ascontiguousarray(uvw, dtype=np.float64)
+        n = self.ns[m]
+        assert uvw.shape == (3, n, n, n)
+        _check(lib().dns_batch_load_velocity(self._b, m, _ptr(uvw)), "dns_batch_load_velocity")
+
+    def step(self, nsteps=1):
+        _check(lib().dns_batch_step(self._b, int(nsteps)), "dns_batch_step")
+
+    def diagnostics(self, m):
+        e, z = ctypes.c_double(), ctypes.c_double()
+        _check(lib().dns_batch_diagnostics(self._b, m, ctypes.byref(e), ctypes.byref(z)), "dns_batch_diagnostics")
+        return e.value, z.value
+
+    def get_spectral(self, m):
+        n = self.ns[m]
+        out = np.empty((3, n, n, n // 2 + 1), dtype=np.complex128)
+        _check(lib().dns_batch_get_spectral(self._b, m, _ptr(out)), "dns_batch_get_spectral")
+        return out
+
+    def get_stale_velocity(self, m):
+        n = self.ns[m]
+        out = np.empty((3, n, n, n), dtype=np.float64)
+        _check(lib().dns_batch_get_stale_velocity(self._b, m, _ptr(out)), "dns_batch_get_stale_velocity")
         return out

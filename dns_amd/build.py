@@ -71,7 +71,8 @@ def build_tools(verbose=False):
     _run([CC, *cflags, os.path.join(PKG, "cli", "dns.c"), f"-L{PKG}", "-ldns_hip",
           "-Wl,-rpath,$ORIGIN/..", "-lm", "-o", dns])
     ic = os.path.join(BIN, "dns-ic")
-    _run([CC, *cflags, os.path.join(PKG, "cli", "dns_ic.c"), "-lm", "-o", ic])
+    _run([CC, *cflags, os.path.join(PKG, "cli", "dns_ic.c"), f"-L{PKG}", "-ldns_hip",
+          "-Wl,-rpath,$ORIGIN/..", "-lm", "-o", ic])
     if verbose:
         print("built", dns, ic)
     return dns

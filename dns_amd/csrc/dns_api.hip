@@ -3,6 +3,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <new>
 #include "engine.hpp"
 
@@ -59,36 +60,13 @@ static int alloc(dns_ctx *c, void **p, size_t bytes) {
   return DNS_OK;
 }
 
-#define DNS_TRY(x)      \
-  do {                  \
-    int rc_ = (x);      \
-    if (rc_) {          \
-      dns_destroy(c);   \
-      return rc_;       \
-    }                   \
-  } while (0)
-
-extern "C" {
-
-const char *dns_strerror(int code) {
-  switch (code) {
-    case DNS_OK: return "success";
-    case DNS_EINVAL: return "invalid argument";
-    case DNS_ENODEV: return "no usable GPU";
-    case DNS_ENOMEM: return "out of device memory";
-    case DNS_EHIP: return "HIP runtime error";
-    case DNS_ENONFINITE: return "non-finite energy or enstrophy";
-    default: return "unknown error";
-  }
-}
-
-const char *dns_last_error(void) { return g_err; }
-
-int dns_create(dns_ctx **out, int n, double nu, double dt, int precision, int device) {
-  if (!out) return DNS_EINVAL;
+// A context of `members` simulations of grid n (members >= 1).
+static int ctx_create(dns_ctx **out, int n, int members, const double *nu, const double *dt, int precision,
+                      int device) {
   *out = nullptr;
   g_err[0] = 0;
   if (precision != DNS_FP64 && precision != DNS_MIXED) return DNS_EINVAL;
+  if (members < 1) return DNS_EINVAL;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
     snprintf(g_err, sizeof g_err, "no HIP device found");
@@ -111,13 +89,19 @@ int dns_create(dns_ctx **out, int n, double nu, double dt, int precision, int de
   c->nfp = (c->nf + 15) / 16 * 16;
   c->prec = precision;
   c->device = device;
-  c->nu = nu;
-  c->dt = dt;
+  c->members = members;
+  c->nu.assign(nu, nu + members);
+  c->dt.assign(dt, dt + members);
   c->kmax = 2. / 3. * (n / 2 + 1);        // main.c:231
   c->invn3 = 1.0 / ((long)n * n * n);     // main.c:188
   c->g.nfp = c->nfp;
   c->g.fs = (long)n * n * c->nfp;
   c->g.rs = (long)n * n * n;
+  c->g.nk = 0;  // wavenumbers m >= 0 inside the dealias box, |m| < kmax (main.c:237-238)
+  while (c->g.nk <= n / 2 && (double)c->g.nk < c->kmax) c->g.nk++;
+  c->g.mW = 6 * c->g.fs;
+  c->g.mS = 3 * c->g.fs;
+  c->g.mR = 3 * c->g.rs;
   if (hipSetDevice(device) != hipSuccess) {
     delete c->impl;
     delete c;
@@ -128,15 +112,31 @@ int dns_create(dns_ctx **out, int n, double nu, double dt, int precision, int de
     delete c;
     return DNS_EHIP;
   }
-  const size_t C64 = sizeof(double2) * c->g.fs;
-  DNS_TRY(alloc(c, &c->uhn, 3 * C64));
-  DNS_TRY(alloc(c, &c->uhr, 3 * impl->sr_size() * c->g.fs));
-  DNS_TRY(alloc(c, &c->acc, 3 * C64));
-  DNS_TRY(alloc(c, &c->W, 6 * 2 * impl->r_size() * c->g.fs));
-  DNS_TRY(alloc(c, &c->us, 3 * impl->r_size() * c->g.rs));
+#define DNS_TRY(x)     \
+  do {                 \
+    int rc_ = (x);     \
+    if (rc_) {         \
+      dns_destroy(c);  \
+      return rc_;      \
+    }                  \
+  } while (0)
+  const size_t M = (size_t)members, C64 = sizeof(double2) * c->g.fs;
+  DNS_TRY(alloc(c, &c->uhn, M * 3 * C64));
+  DNS_TRY(alloc(c, &c->uhr, M * 3 * impl->sr_size() * c->g.fs));
+  DNS_TRY(alloc(c, &c->acc, M * 3 * C64));
+  DNS_TRY(alloc(c, &c->W, M * 6 * 2 * impl->r_size() * c->g.fs));
+  DNS_TRY(alloc(c, &c->us, M * 3 * impl->r_size() * c->g.rs));
   DNS_TRY(alloc(c, (void **)&c->part, sizeof(double) * 4 * DIAG_BLOCKS));
   DNS_TRY(alloc(c, (void **)&c->dres, sizeof(double) * 4));
+  DNS_TRY(alloc(c, (void **)&c->nudt, sizeof(double2) * M));
+  std::vector<double2> nd(M);
+  for (size_t m = 0; m < M; m++) nd[m] = make_double2(nu[m], dt[m]);
+  if (hipMemcpyAsync(c->nudt, nd.data(), sizeof(double2) * M, hipMemcpyHostToDevice, c->stream) != hipSuccess) {
+    dns_destroy(c);
+    return DNS_EHIP;
+  }
   DNS_TRY(impl->make_twiddles(c));
+#undef DNS_TRY
   if (hipStreamSynchronize(c->stream) != hipSuccess) {
     dns_destroy(c);
     return DNS_EHIP;
@@ -145,11 +145,84 @@ int dns_create(dns_ctx **out, int n, double nu, double dt, int precision, int de
   return DNS_OK;
 }
 
+static int ctx_diagnostics(dns_ctx *c, int m, double *energy, double *enstrophy) {
+  DNS_HIP(hipSetDevice(c->device));
+  const int nb = (int)((long)c->n * c->n < DIAG_BLOCKS ? (long)c->n * c->n : DIAG_BLOCKS);
+  const double2 *uh = (const double2 *)c->uhn + m * c->g.mS;
+  k_diag<256><<<nb, 256, 0, c->stream>>>(uh, c->n, c->g, c->part);
+  DNS_LAUNCHED();
+  k_diag_final<256><<<1, 256, 0, c->stream>>>(c->part, nb, c->dres);
+  DNS_LAUNCHED();
+  double r[4];
+  DNS_HIP(hipMemcpyAsync(r, c->dres, sizeof r, hipMemcpyDeviceToHost, c->stream));
+  DNS_HIP(hipStreamSynchronize(c->stream));
+  // main.c:265-266: scale by (1/n^3)^2 after the sum
+  const double s = c->invn3 * c->invn3;
+  *energy = (r[0] + r[1]) * s;
+  *enstrophy = (r[2] + r[3]) * s;
+  if (!std::isfinite(*energy) || !std::isfinite(*enstrophy)) return DNS_ENONFINITE;
+  return DNS_OK;
+}
+
+static int ctx_get_spectral(dns_ctx *c, int m, double *uvw_hat) {
+  DNS_HIP(hipSetDevice(c->device));
+  const double2 *src = (const double2 *)c->uhn + m * c->g.mS;
+  DNS_HIP(hipMemcpy2DAsync(uvw_hat, sizeof(double2) * c->nf, src, sizeof(double2) * c->nfp,
+                           sizeof(double2) * c->nf, (size_t)3 * c->n * c->n, hipMemcpyDeviceToHost, c->stream));
+  DNS_HIP(hipStreamSynchronize(c->stream));
+  return DNS_OK;
+}
+
+static int ctx_set_spectral(dns_ctx *c, int m, const double *uvw_hat) {
+  DNS_HIP(hipSetDevice(c->device));
+  c->w_ready = 0;
+  double2 *dst = (double2 *)c->uhn + m * c->g.mS;
+  DNS_HIP(hipMemcpy2DAsync(dst, sizeof(double2) * c->nfp, uvw_hat, sizeof(double2) * c->nf,
+                           sizeof(double2) * c->nf, (size_t)3 * c->n * c->n, hipMemcpyHostToDevice, c->stream));
+  DNS_HIP(hipStreamSynchronize(c->stream));
+  return DNS_OK;
+}
+
+// A batch: one context per distinct grid size, members mapped to
+// (context, index).  Steps are replayed from one captured graph that runs the
+// contexts on concurrent branches.
+struct dns_batch {
+  int members, device;
+  std::vector<dns_ctx *> groups;
+  std::vector<int> group_of, index_of;
+  hipStream_t main;
+  hipEvent_t fork;
+  std::vector<hipEvent_t> join;
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+};
+
+extern "C" {
+
+const char *dns_strerror(int code) {
+  switch (code) {
+    case DNS_OK: return "success";
+    case DNS_EINVAL: return "invalid argument";
+    case DNS_ENODEV: return "no usable GPU";
+    case DNS_ENOMEM: return "out of device memory";
+    case DNS_EHIP: return "HIP runtime error";
+    case DNS_ENONFINITE: return "non-finite energy or enstrophy";
+    default: return "unknown error";
+  }
+}
+
+const char *dns_last_error(void) { return g_err; }
+
+int dns_create(dns_ctx **out, int n, double nu, double dt, int precision, int device) {
+  if (!out) return DNS_EINVAL;
+  return ctx_create(out, n, 1, &nu, &dt, precision, device);
+}
+
 void dns_destroy(dns_ctx *c) {
   if (!c) return;
   hipSetDevice(c->device);
   if (c->stream) hipStreamSynchronize(c->stream);
-  void *ptrs[] = {c->uhn, c->uhr, c->acc, c->W, c->us, c->phat, c->tw64, c->tw32, c->part, c->dres};
+  void *ptrs[] = {c->uhn, c->uhr, c->acc, c->W, c->us, c->phat, c->tw64, c->tw32, c->part, c->dres, c->nudt};
   for (void *p : ptrs)
     if (p) hipFree(p);
   for (auto &t : c->ev_pending) {
@@ -171,7 +244,7 @@ int dns_load_velocity(dns_ctx *c, const double *uvw) {
   if (!c || !uvw) return DNS_EINVAL;
   DNS_HIP(hipSetDevice(c->device));
   c->have_phat = 0;
-  return c->impl->load(c, uvw);
+  return c->impl->load(c, 0, uvw);
 }
 
 int dns_step(dns_ctx *c, long nsteps) {
@@ -194,57 +267,35 @@ int dns_set_pressure_capture(dns_ctx *c, int on) {
 
 int dns_diagnostics(dns_ctx *c, double *energy, double *enstrophy) {
   if (!c || !energy || !enstrophy) return DNS_EINVAL;
-  DNS_HIP(hipSetDevice(c->device));
-  const int nb = (int)((long)c->n * c->n < DIAG_BLOCKS ? (long)c->n * c->n : DIAG_BLOCKS);
-  k_diag<256><<<nb, 256, 0, c->stream>>>((const double2 *)c->uhn, c->n, c->g, c->part);
-  DNS_LAUNCHED();
-  k_diag_final<256><<<1, 256, 0, c->stream>>>(c->part, nb, c->dres);
-  DNS_LAUNCHED();
-  double r[4];
-  DNS_HIP(hipMemcpyAsync(r, c->dres, sizeof r, hipMemcpyDeviceToHost, c->stream));
-  DNS_HIP(hipStreamSynchronize(c->stream));
-  // main.c:265-266: scale by (1/n^3)^2 after the sum
-  const double s = c->invn3 * c->invn3;
-  *energy = (r[0] + r[1]) * s;
-  *enstrophy = (r[2] + r[3]) * s;
-  if (!std::isfinite(*energy) || !std::isfinite(*enstrophy)) return DNS_ENONFINITE;
-  return DNS_OK;
+  return ctx_diagnostics(c, 0, energy, enstrophy);
 }
 
 int dns_snapshot(dns_ctx *c, double *uvwp) {
   if (!c || !uvwp) return DNS_EINVAL;
   DNS_HIP(hipSetDevice(c->device));
-  return c->impl->snapshot(c, uvwp);
+  return c->impl->snapshot(c, 0, uvwp);
 }
 
 int dns_get_spectral(dns_ctx *c, double *uvw_hat) {
   if (!c || !uvw_hat) return DNS_EINVAL;
-  DNS_HIP(hipSetDevice(c->device));
-  DNS_HIP(hipMemcpy2DAsync(uvw_hat, sizeof(double2) * c->nf, c->uhn, sizeof(double2) * c->nfp,
-                           sizeof(double2) * c->nf, (size_t)3 * c->n * c->n, hipMemcpyDeviceToHost, c->stream));
-  DNS_HIP(hipStreamSynchronize(c->stream));
-  return DNS_OK;
+  return ctx_get_spectral(c, 0, uvw_hat);
 }
 
 int dns_set_spectral(dns_ctx *c, const double *uvw_hat) {
   if (!c || !uvw_hat) return DNS_EINVAL;
-  DNS_HIP(hipSetDevice(c->device));
-  DNS_HIP(hipMemcpy2DAsync(c->uhn, sizeof(double2) * c->nfp, uvw_hat, sizeof(double2) * c->nf,
-                           sizeof(double2) * c->nf, (size_t)3 * c->n * c->n, hipMemcpyHostToDevice, c->stream));
-  DNS_HIP(hipStreamSynchronize(c->stream));
-  return DNS_OK;
+  return ctx_set_spectral(c, 0, uvw_hat);
 }
 
 int dns_get_stale_velocity(dns_ctx *c, double *uvw) {
   if (!c || !uvw) return DNS_EINVAL;
   DNS_HIP(hipSetDevice(c->device));
-  return c->impl->get_us(c, uvw);
+  return c->impl->get_us(c, 0, uvw);
 }
 
 int dns_set_stale_velocity(dns_ctx *c, const double *uvw) {
   if (!c || !uvw) return DNS_EINVAL;
   DNS_HIP(hipSetDevice(c->device));
-  return c->impl->set_us(c, uvw);
+  return c->impl->set_us(c, 0, uvw);
 }
 
 int dns_set_kernel_timing(dns_ctx *c, int on) {
@@ -286,10 +337,11 @@ int dns_kernel_timing(dns_ctx *c, double *ms, long *launches) {
 }
 
 const char *dns_kernel_slot_name(int slot) {
-  static const char *pass[5] = {"xinv_curl", "ypass_inv", "zfused", "ypass_fwd", "xfwd_rk"};
+  static const char *pass[DNS_NPASS] = {"xinv_curl", "ypass_inv", "zfused", "ypass_fwd", "xfwd_rk", "xfwd_rk_xinv"};
   static char names[DNS_NSLOT][32];
   if (slot < 0 || slot >= DNS_NSLOT) return "";
-  if (!names[slot][0]) snprintf(names[slot], sizeof names[slot], "%s.s%d", pass[slot % 5], slot / 5 + 1);
+  if (!names[slot][0])
+    snprintf(names[slot], sizeof names[slot], "%s.s%d", pass[slot % DNS_NPASS], slot / DNS_NPASS + 1);
   return names[slot];
 }
 
@@ -305,4 +357,154 @@ int dns_irfft3(dns_ctx *c, const double *spec, double *real) {
   return c->impl->irfft3(c, spec, real);
 }
 
+
+// ---------------------------------------------------------------------------
+// Batched ensembles.
+
+void dns_batch_destroy(dns_batch *b) {
+  if (!b) return;
+  hipSetDevice(b->device);
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  if (b->main) hipStreamSynchronize(b->main);
+  for (dns_ctx *c : b->groups) dns_destroy(c);
+  for (hipEvent_t e : b->join) hipEventDestroy(e);
+  if (b->fork) hipEventDestroy(b->fork);
+  if (b->main) hipStreamDestroy(b->main);
+  delete b;
+}
+
+int dns_batch_create(dns_batch **out, int members, const int *n, const double *nu, const double *dt,
+                     int precision, int device) {
+  if (!out || members < 1 || !n || !nu || !dt) return DNS_EINVAL;
+  *out = nullptr;
+  dns_batch *b = new (std::nothrow) dns_batch();
+  if (!b) return DNS_ENOMEM;
+  b->members = members;
+  b->device = device;
+  b->group_of.assign(members, -1);
+  b->index_of.assign(members, -1);
+  std::vector<int> sizes;
+  for (int m = 0; m < members; m++)
+    if (std::find(sizes.begin(), sizes.end(), n[m]) == sizes.end()) sizes.push_back(n[m]);
+  for (size_t gi = 0; gi < sizes.size(); gi++) {
+    std::vector<double> gnu, gdt;
+    for (int m = 0; m < members; m++)
+      if (n[m] == sizes[gi]) {
+        b->group_of[m] = (int)gi;
+        b->index_of[m] = (int)gnu.size();
+        gnu.push_back(nu[m]);
+        gdt.push_back(dt[m]);
+      }
+    dns_ctx *c = nullptr;
+    int rc = ctx_create(&c, sizes[gi], (int)gnu.size(), gnu.data(), gdt.data(), precision, device);
+    if (rc) {
+      dns_batch_destroy(b);
+      return rc;
+    }
+    b->groups.push_back(c);
+  }
+  if (hipStreamCreateWithFlags(&b->main, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess) {
+    dns_batch_destroy(b);
+    return DNS_EHIP;
+  }
+  b->join.resize(b->groups.size());
+  for (auto &e : b->join)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+      dns_batch_destroy(b);
+      return DNS_EHIP;
+    }
+  *out = b;
+  return DNS_OK;
+}
+
+static int batch_member(dns_batch *b, int m, dns_ctx **c, int *i) {
+  if (!b || m < 0 || m >= b->members) return DNS_EINVAL;
+  *c = b->groups[b->group_of[m]];
+  *i = b->index_of[m];
+  return DNS_OK;
+}
+
+int dns_batch_load_velocity(dns_batch *b, int member, const double *uvw) {
+  dns_ctx *c;
+  int i, rc = batch_member(b, member, &c, &i);
+  if (rc || !uvw) return rc ? rc : DNS_EINVAL;
+  DNS_HIP(hipSetDevice(c->device));
+  return c->impl->load(c, i, uvw);
+}
+
+int dns_batch_step(dns_batch *b, long nsteps) {
+  if (!b || nsteps < 0) return DNS_EINVAL;
+  if (nsteps == 0) return DNS_OK;
+  DNS_HIP(hipSetDevice(b->device));
+  // stage-1 partials of every group must be present before replaying
+  for (dns_ctx *c : b->groups) {
+    if (c->timing) return DNS_EINVAL;
+    int rc = c->impl->prepare(c);
+    if (rc) return rc;
+    DNS_HIP(hipStreamSynchronize(c->stream));
+  }
+  if (!b->exec) {  // capture one step of all groups, one branch per group
+    DNS_HIP(hipStreamBeginCapture(b->main, hipStreamCaptureModeGlobal));
+    DNS_HIP(hipEventRecord(b->fork, b->main));
+    for (size_t gi = 0; gi < b->groups.size(); gi++) {
+      dns_ctx *c = b->groups[gi];
+      DNS_HIP(hipStreamWaitEvent(c->stream, b->fork, 0));
+      int rc = c->impl->step(c, 1);
+      if (rc) {
+        hipGraph_t g;
+        hipStreamEndCapture(b->main, &g);
+        if (g) hipGraphDestroy(g);
+        return rc;
+      }
+      DNS_HIP(hipEventRecord(b->join[gi], c->stream));
+      DNS_HIP(hipStreamWaitEvent(b->main, b->join[gi], 0));
+    }
+    DNS_HIP(hipStreamEndCapture(b->main, &b->graph));
+    DNS_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  }
+  for (long s = 0; s < nsteps; s++) DNS_HIP(hipGraphLaunch(b->exec, b->main));
+  // the groups' own streams must see the graph's work before later calls
+  DNS_HIP(hipEventRecord(b->fork, b->main));
+  for (dns_ctx *c : b->groups) DNS_HIP(hipStreamWaitEvent(c->stream, b->fork, 0));
+  return DNS_OK;
+}
+
+int dns_batch_diagnostics(dns_batch *b, int member, double *energy, double *enstrophy) {
+  dns_ctx *c;
+  int i, rc = batch_member(b, member, &c, &i);
+  if (rc || !energy || !enstrophy) return rc ? rc : DNS_EINVAL;
+  return ctx_diagnostics(c, i, energy, enstrophy);
+}
+
+int dns_batch_get_spectral(dns_batch *b, int member, double *uvw_hat) {
+  dns_ctx *c;
+  int i, rc = batch_member(b, member, &c, &i);
+  if (rc || !uvw_hat) return rc ? rc : DNS_EINVAL;
+  return ctx_get_spectral(c, i, uvw_hat);
+}
+
+int dns_batch_get_stale_velocity(dns_batch *b, int member, double *uvw) {
+  dns_ctx *c;
+  int i, rc = batch_member(b, member, &c, &i);
+  if (rc || !uvw) return rc ? rc : DNS_EINVAL;
+  DNS_HIP(hipSetDevice(c->device));
+  return c->impl->get_us(c, i, uvw);
+}
+
+int dns_batch_member_n(dns_batch *b, int member) {
+  dns_ctx *c;
+  int i, rc = batch_member(b, member, &c, &i);
+  return rc ? rc : c->n;
+}
+
+long long dns_batch_device_bytes(const dns_batch *b) {
+  long long s = 0;
+  if (b)
+    for (dns_ctx *c : b->groups) s += c->bytes;
+  return s;
+}
+
 }  // extern "C"
+

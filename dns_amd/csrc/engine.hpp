@@ -1,7 +1,13 @@
 // Context and per-grid-size engine of libdns_hip.
 //
-// Device memory plan (U = one spectral field, n*n*nfp complex; R = one
-// physical field, n^3 reals):
+// A context holds M >= 1 simulations ("members") of the same grid size n
+// on one stream; every buffer below is repeated per member (member stride
+// in Geo) and every step kernel covers all members through blockIdx.z.
+// The single-simulation API uses M = 1; a batch (dns_batch_*) owns one
+// context per distinct n.
+//
+// Device memory plan per member (U = one spectral field, n*n*nfp complex;
+// R = one physical field, n^3 reals):
 //   uhn  3U double  spectral state at the start of the step (Û_n); stage 4
 //                   writes Û_{n+1} here
 //   uhr  3U SR      stage state Û_r (the argument of stages 2..4)
@@ -41,18 +47,24 @@ struct Impl;
 
 }  // namespace dnsk
 
-// Kernel timing slots: 5 * stage + pass, pass A..E = 0..4 (see kernels.hpp).
-enum { DNS_NSLOT = 20 };
+// Kernel timing slots: DNS_NPASS * stage + pass; passes 0..4 = A..E of
+// kernels.hpp, 5 = E fused with the next stage's A.
+enum { DNS_NPASS = 6, DNS_NSLOT = 4 * DNS_NPASS };
 
 struct dns_ctx {
-  int n, nf, nfp, prec, device;
-  double nu, dt, kmax, invn3;
+  int n, nf, nfp, prec, device, members;
+  double kmax, invn3;
+  std::vector<double> nu, dt;  // per member
   hipStream_t stream;
   dnsk::Geo g;
   void *uhn, *uhr, *acc, *W, *us, *phat;
   void *tw64, *tw32;
+  double2 *nudt;  // device copy of (nu, dt) per member
   double *part, *dres;
   int capture, have_phat;
+  // W holds the omega partials of the next step's stage 1 (written by the
+  // fused E+A of the last stage 4); cleared whenever W or the state change
+  int w_ready;
   long long bytes;
   dnsk::Impl *impl;
   // HIP-event timing of the step's kernels on `stream` (off by default)
@@ -74,13 +86,17 @@ struct Impl {
   virtual size_t sr_size() const = 0;  // bytes per complex element of uhr
   virtual size_t r_size() const = 0;   // bytes per real element of W/us
   virtual int make_twiddles(dns_ctx *c) = 0;
-  virtual int load(dns_ctx *c, const double *uvw) = 0;
+  virtual int load(dns_ctx *c, int m, const double *uvw) = 0;
+  // Pass A of stage 1 for every member unless the partials are ready.
+  virtual int prepare(dns_ctx *c) = 0;
+  // nsteps steps of all members; with `ready` the partials of stage 1 are
+  // taken as present (graph capture after prepare()).
   virtual int step(dns_ctx *c, long nsteps) = 0;
-  virtual int snapshot(dns_ctx *c, double *uvwp) = 0;
+  virtual int snapshot(dns_ctx *c, int m, double *uvwp) = 0;
   virtual int rfft3(dns_ctx *c, const double *in, double *out) = 0;
   virtual int irfft3(dns_ctx *c, const double *in, double *out) = 0;
-  virtual int get_us(dns_ctx *c, double *out) = 0;
-  virtual int set_us(dns_ctx *c, const double *in) = 0;
+  virtual int get_us(dns_ctx *c, int m, double *out) = 0;
+  virtual int set_us(dns_ctx *c, int m, const double *in) = 0;
 };
 
 // Event bracketing of one launch when timing is on.
@@ -144,8 +160,9 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
   // Column-tile widths: 128-byte row segments, at most 1024 threads.
   static constexpr int T_R = (sizeof(R) == 8 ? 8 : 16) * TPP > 1024 ? 1024 / TPP : (sizeof(R) == 8 ? 8 : 16);
   static constexpr int T_D = 8 * TPP > 1024 ? 1024 / TPP : 8;
-  // z pencils per block: about 256 threads.
-  static constexpr int PPW = TPP >= 256 ? 1 : 256 / TPP;
+  // z pencils per block: one wave (or one pencil when a pencil needs more
+  // than a wave), so the pass's barriers never couple independent pencils.
+  static constexpr int PPW = TPP >= 64 ? 1 : 64 / TPP;
   static constexpr int NF = N / 2 + 1;
 
   size_t sr_size() const override { return sizeof(cx<SR>); }
@@ -154,7 +171,10 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
   static const cx<R> *twR(dns_ctx *c) { return (const cx<R> *)(sizeof(R) == 8 ? c->tw64 : c->tw32); }
   static const cx<double> *twD(dns_ctx *c) { return (const cx<double> *)c->tw64; }
 
-  dim3 grid_x(int T) const { return dim3((NF + T - 1) / T, N); }
+  dim3 grid_x(int T, int m = 1) const { return dim3((NF + T - 1) / T, N, m); }
+
+  // Member m's slice of the W scratch, as bytes.
+  static char *wslice(dns_ctx *c, int m) { return (char *)c->W + (size_t)m * c->g.mW * sizeof(cx<R>); }
 
   int make_twiddles(dns_ctx *c) override {
     auto t = twiddles_ld<N>();
@@ -172,8 +192,8 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
     return DNS_OK;
   }
 
-  // Forward 3-D transform (double) of `nf` physical fields at `phys` into
-  // spectral fields at `spec` (in the padded layout).
+  // Forward 3-D transform (double) of `nfields` physical fields at `phys`
+  // into spectral fields at `spec` (padded layout), one member.
   int fwd3_double(dns_ctx *c, const double *phys, double2 *spec, int nfields) {
     hipStream_t s = c->stream;
     k_zr2c<N, double, PPW><<<N * N / PPW, PPW * TPP, 0, s>>>(phys, spec, nfields, c->g, twD(c));
@@ -185,26 +205,38 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
     return DNS_OK;
   }
 
-  int load(dns_ctx *c, const double *uvw) override {
+  int load(dns_ctx *c, int m, const double *uvw) override {
+    c->w_ready = 0;
     hipStream_t s = c->stream;
-    double *tmp = (double *)c->W;  // 3 physical double fields fit in W
+    double *tmp = (double *)wslice(c, m);  // 3 physical double fields fit in a member's W
     const size_t rb = sizeof(double) * c->g.rs;
     DNS_HIP(hipMemcpyAsync(tmp, uvw, 3 * rb, hipMemcpyHostToDevice, s));
-    int rc = fwd3_double(c, tmp, (double2 *)c->uhn, 3);
+    int rc = fwd3_double(c, tmp, (double2 *)c->uhn + m * c->g.mS, 3);
     if (rc) return rc;
+    R *us = (R *)c->us + m * c->g.mR;
     if (sizeof(R) == 8) {
-      DNS_HIP(hipMemcpyAsync(c->us, tmp, 3 * rb, hipMemcpyDeviceToDevice, s));
+      DNS_HIP(hipMemcpyAsync(us, tmp, 3 * rb, hipMemcpyDeviceToDevice, s));
     } else {
-      k_convert<double, R><<<4096, 256, 0, s>>>(tmp, (R *)c->us, 3 * c->g.rs);
+      k_convert<double, R><<<4096, 256, 0, s>>>(tmp, us, 3 * c->g.rs);
       DNS_LAUNCHED();
     }
     DNS_HIP(hipStreamSynchronize(s));
     return DNS_OK;
   }
 
+  int prepare(dns_ctx *c) override {
+    if (c->w_ready) return DNS_OK;
+    const dim3 gx = grid_x(T_R, c->members), bx(T_R * TPP);
+    DNS_TIMED(c, 0, (k_xinv_curl<N, R, T_R, false, double><<<gx, bx, 0, c->stream>>>(
+                        (const double2 *)c->uhn, (cx<R> *)c->W, c->g, twR(c))));
+    c->w_ready = 1;
+    return DNS_OK;
+  }
+
   int step(dns_ctx *c, long nsteps) override {
     hipStream_t s = c->stream;
     const Geo g = c->g;
+    const int M = c->members;
     cx<R> *W = (cx<R> *)c->W;
     cx<SR> *uhr = (cx<SR> *)c->uhr;
     double2 *uhn = (double2 *)c->uhn, *acc = (double2 *)c->acc;
@@ -212,48 +244,53 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
     const R invn3 = (R)c->invn3;
     static const double a_rk[4] = {1 / 6.0, 1 / 3.0, 1 / 3.0, 1 / 6.0};  // main.c:15
     static const double b_rk[4] = {0.5, 0.5, 1.0, 0.0};                 // main.c:16
-    const dim3 gx = grid_x(T_R), bx(T_R * TPP);
-    const dim3 gz(N * N / PPW), bz(PPW * TPP);
+    const dim3 gx = grid_x(T_R, M), bx(T_R * TPP);
+    const dim3 gz(N * N / PPW, 1, M), bz(PPW * TPP);
+    const dim3 gy_trunc((g.nk + T_R - 1) / T_R, N, M);  // forward y pass: dealiased kz columns only
+    int rc = prepare(c);
+    if (rc) return rc;
     for (long st = 0; st < nsteps; st++) {
       double2 *ph = (c->capture && st == nsteps - 1) ? (double2 *)c->phat : nullptr;
       for (int rk = 0; rk < 4; rk++) {
-        RKParams q{c->dt, c->nu, c->kmax, a_rk[rk], b_rk[rk]};
-        const int sl = 5 * rk;
+        RKParams q{c->nudt, c->kmax, a_rk[rk], b_rk[rk]};
+        const int sl = DNS_NPASS * rk;
+        // pass A of this stage ran fused into the previous stage's E
         if (rk == 0) {
-          DNS_TIMED(c, sl + 0, (k_xinv_curl<N, R, T_R, false, double><<<gx, bx, 0, s>>>(uhn, W, g, tw)));
-          DNS_TIMED(c, sl + 1, (k_ypass<N, R, T_R, true><<<gx, bx, 0, s>>>(W + 3 * g.fs, 3, g, tw)));
+          DNS_TIMED(c, sl + 1, (k_ypass<N, R, T_R, true><<<gx, bx, 0, s>>>(W + 3 * g.fs, 3, g, tw, g.mW)));
           DNS_TIMED(c, sl + 2, (k_zfused<N, R, PPW, 0, R><<<gz, bz, 0, s>>>(W, (R *)c->us, g, invn3, tw)));
         } else {
-          DNS_TIMED(c, sl + 0, (k_xinv_curl<N, R, T_R, true, SR><<<gx, bx, 0, s>>>(uhr, W, g, tw)));
-          DNS_TIMED(c, sl + 1, (k_ypass<N, R, T_R, true><<<gx, bx, 0, s>>>(W, 6, g, tw)));
+          DNS_TIMED(c, sl + 1, (k_ypass<N, R, T_R, true><<<gx, bx, 0, s>>>(W, 6, g, tw, g.mW)));
           if (rk < 3)
             DNS_TIMED(c, sl + 2, (k_zfused<N, R, PPW, 1, R><<<gz, bz, 0, s>>>(W, (R *)c->us, g, invn3, tw)));
           else
             DNS_TIMED(c, sl + 2, (k_zfused<N, R, PPW, 2, R><<<gz, bz, 0, s>>>(W, (R *)c->us, g, invn3, tw)));
         }
-        DNS_TIMED(c, sl + 3, (k_ypass<N, R, T_R, false><<<gx, bx, 0, s>>>(W, 3, g, tw)));
+        DNS_TIMED(c, sl + 3,
+                  (k_ypass<N, R, T_R, false, 1, false, true><<<gy_trunc, bx, 0, s>>>(W, 3, g, tw, g.mW)));
+        // E fused with the next stage's A (the next step's stage 1 after rk 3)
         switch (rk) {
           case 0:
-            DNS_TIMED(c, sl + 4, (k_xfwd_rk<N, R, T_R, 0, SR><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, nullptr, g, q, tw)));
+            DNS_TIMED(c, sl + 5, (k_xfwd_rk_xinv<N, R, T_R, 0, SR><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, nullptr, g, q, tw)));
             break;
           case 1:
-            DNS_TIMED(c, sl + 4, (k_xfwd_rk<N, R, T_R, 1, SR><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, nullptr, g, q, tw)));
+            DNS_TIMED(c, sl + 5, (k_xfwd_rk_xinv<N, R, T_R, 1, SR><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, nullptr, g, q, tw)));
             break;
           case 2:
-            DNS_TIMED(c, sl + 4, (k_xfwd_rk<N, R, T_R, 2, SR><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, nullptr, g, q, tw)));
+            DNS_TIMED(c, sl + 5, (k_xfwd_rk_xinv<N, R, T_R, 2, SR><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, nullptr, g, q, tw)));
             break;
           default:
-            DNS_TIMED(c, sl + 4, (k_xfwd_rk<N, R, T_R, 3, SR><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, ph, g, q, tw)));
+            DNS_TIMED(c, sl + 5, (k_xfwd_rk_xinv<N, R, T_R, 3, SR><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, ph, g, q, tw)));
             break;
         }
       }
       if (ph) c->have_phat = 1;
     }
+    c->w_ready = 1;
     return DNS_OK;
   }
 
-  // c2r (double) of one spectral double field into a physical double
-  // field times `scale`; uses W as scratch beyond `phys_out` if needed.
+  // c2r (double) of one spectral double field into a physical double field
+  // times `scale`, one member.
   int inv3_double(dns_ctx *c, const double2 *spec, double2 *tmpc, double *phys, double scale) {
     hipStream_t s = c->stream;
     k_xplain<N, double, T_D, true, double><<<grid_x(T_D), T_D * TPP, 0, s>>>(spec, tmpc, 1, c->g, twD(c));
@@ -265,17 +302,19 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
     return DNS_OK;
   }
 
-  int snapshot(dns_ctx *c, double *uvwp) override {
+  int snapshot(dns_ctx *c, int m, double *uvwp) override {
+    c->w_ready = 0;
     hipStream_t s = c->stream;
-    double2 *tmpc = (double2 *)c->W;
-    double *tmpr = (double *)((char *)c->W + sizeof(double2) * c->g.fs);
+    double2 *tmpc = (double2 *)wslice(c, m);
+    double *tmpr = (double *)((char *)tmpc + sizeof(double2) * c->g.fs);
     const size_t rb = sizeof(double) * c->g.rs;
     for (int f = 0; f < 4; f++) {
       if (f == 3 && !c->have_phat) {
         memset(uvwp + 3 * c->g.rs, 0, rb);
         continue;
       }
-      const double2 *src = f < 3 ? (const double2 *)c->uhn + f * c->g.fs : (const double2 *)c->phat;
+      const double2 *src = f < 3 ? (const double2 *)c->uhn + m * c->g.mS + f * c->g.fs
+                                 : (const double2 *)c->phat + m * c->g.fs;
       int rc = inv3_double(c, src, tmpc, tmpr, c->invn3);
       if (rc) return rc;
       DNS_HIP(hipMemcpyAsync(uvwp + f * c->g.rs, tmpr, rb, hipMemcpyDeviceToHost, s));
@@ -285,6 +324,7 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
   }
 
   int rfft3(dns_ctx *c, const double *in, double *out) override {
+    c->w_ready = 0;
     hipStream_t s = c->stream;
     const Geo g = c->g;
     double *phys = (double *)c->W;
@@ -309,6 +349,7 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
   }
 
   int irfft3(dns_ctx *c, const double *in, double *out) override {
+    c->w_ready = 0;
     hipStream_t s = c->stream;
     const Geo g = c->g;
     // W: [spectral double input | complex R partial | physical double output]
@@ -329,13 +370,15 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
     return DNS_OK;
   }
 
-  int get_us(dns_ctx *c, double *out) override {
+  int get_us(dns_ctx *c, int m, double *out) override {
     hipStream_t s = c->stream;
     const size_t cnt = 3 * (size_t)c->g.rs;
-    const double *src = (const double *)c->us;
+    const R *us = (const R *)c->us + m * c->g.mR;
+    const double *src = (const double *)us;
     if (sizeof(R) != 8) {
-      double *tmp = (double *)c->W;
-      k_convert<R, double><<<4096, 256, 0, s>>>((const R *)c->us, tmp, (long)cnt);
+      c->w_ready = 0;
+      double *tmp = (double *)wslice(c, m);
+      k_convert<R, double><<<4096, 256, 0, s>>>(us, tmp, (long)cnt);
       DNS_LAUNCHED();
       src = tmp;
     }
@@ -344,15 +387,17 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
     return DNS_OK;
   }
 
-  int set_us(dns_ctx *c, const double *in) override {
+  int set_us(dns_ctx *c, int m, const double *in) override {
+    c->w_ready = 0;
     hipStream_t s = c->stream;
     const size_t cnt = 3 * (size_t)c->g.rs;
+    R *us = (R *)c->us + m * c->g.mR;
     if (sizeof(R) == 8) {
-      DNS_HIP(hipMemcpyAsync(c->us, in, cnt * sizeof(double), hipMemcpyHostToDevice, s));
+      DNS_HIP(hipMemcpyAsync(us, in, cnt * sizeof(double), hipMemcpyHostToDevice, s));
     } else {
-      double *tmp = (double *)c->W;
+      double *tmp = (double *)wslice(c, m);
       DNS_HIP(hipMemcpyAsync(tmp, in, cnt * sizeof(double), hipMemcpyHostToDevice, s));
-      k_convert<double, R><<<4096, 256, 0, s>>>(tmp, (R *)c->us, (long)cnt);
+      k_convert<double, R><<<4096, 256, 0, s>>>(tmp, us, (long)cnt);
       DNS_LAUNCHED();
     }
     DNS_HIP(hipStreamSynchronize(s));

@@ -42,6 +42,28 @@ template <typename R, bool INV> __device__ __forceinline__ cx<R> rot(cx<R> a) {
   return INV ? mk<R>(-a.y, a.x) : mk<R>(a.y, -a.x);
 }
 
+// Streaming stores: every pass writes each element exactly once and the next
+// pass reads it from HBM, so stores use the non-temporal path.
+#ifndef DNS_NT_STORES
+#define DNS_NT_STORES 1
+#endif
+template <typename R> __device__ __forceinline__ void st(cx<R> *p, cx<R> v) {
+#if DNS_NT_STORES
+  typedef R v2 __attribute__((ext_vector_type(2)));
+  v2 t = {v.x, v.y};
+  __builtin_nontemporal_store(t, (v2 *)p);
+#else
+  *p = v;
+#endif
+}
+template <typename R> __device__ __forceinline__ void st_real(R *p, R v) {
+#if DNS_NT_STORES
+  __builtin_nontemporal_store(v, p);
+#else
+  *p = v;
+#endif
+}
+
 // ---------------------------------------------------------------------------
 // Radix schedules.  E = elements per thread (divisible by every radix).
 template <int N> struct Sched;

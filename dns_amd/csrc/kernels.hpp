@@ -17,7 +17,8 @@
 //                    366-371), u x omega (372-377), r2c along z of N
 //   D  k_ypass     : forward FFT along y
 //   E  k_xfwd_rk   : forward FFT along x + dealias*dt (381-386), projection
-//                    and viscous term (387-399), RK update (400-413)
+//                    and viscous term (387-399), RK update (400-413);
+//                    k_xfwd_rk_xinv fuses it with the next stage's A
 #pragma once
 #include "fft_engine.hpp"
 
@@ -27,11 +28,21 @@ struct Geo {
   int nfp;   // padded row length of spectral fields (complex elements)
   long fs;   // spectral field stride (complex elements) = n*n*nfp
   long rs;   // physical field stride (reals) = n^3
+  int nk;    // wavenumbers m >= 0 with m < kmax: the dealiased box is
+             // |kx|,|ky| < kmax <=> index in [0,nk) or [n-nk+1,n), kz in [0,nk)
+  // Member strides (elements) when one context holds M simulations of the
+  // same n: blockIdx.z selects the member.  W-type partials, spectral state
+  // (uhn/uhr/acc) and stale physical velocity.
+  long mW, mS, mR;
 };
 
+// Index i of an x or y axis lies inside the dealiased box.
+__device__ __forceinline__ bool in_box(int i, int n, int nk) { return i < nk || i > n - nk; }
+
 struct RKParams {
-  double dt, nu, kmax;
-  double a, b;  // a[rk], b[rk] of main.c:15-16 for this stage
+  const double2 *nudt;  // per member (viscosity, time step)
+  double kmax;
+  double a, b;          // a[rk], b[rk] of main.c:15-16 for this stage
 };
 
 template <typename R> __device__ __forceinline__ cx<R> zero() { return mk<R>((R)0, (R)0); }
@@ -47,24 +58,44 @@ template <int N> __device__ __forceinline__ int wave(int i) { return i < N / 2 ?
 // ---------------------------------------------------------------------------
 // B/D: FFT along y of `nfields` consecutive spectral fields, in place.
 // Grid (ceil(nf/T), n): blockIdx.y = i.
-template <int N, typename R, int T, bool INV>
-__global__ __launch_bounds__(T *(N / Sched<N>::E)) void k_ypass(cx<R> *__restrict__ data, int nfields, Geo g,
-                                                                 const cx<R> *__restrict__ tw) {
+// MINW: minimum resident waves per SIMD the register allocation must allow.
+// PIPE: load field f+1 while field f is transformed.
+// TRUNC (forward transform of the nonlinear term only): the columns
+// k >= g.nk are not read and only rows inside the dealiased box are written;
+// the grid then covers ceil(nk/T) column tiles.
+template <int N, typename R, int T, bool INV, int MINW = 1, bool PIPE = false, bool TRUNC = false>
+__global__ __launch_bounds__(T *(N / Sched<N>::E), MINW) void k_ypass(cx<R> *__restrict__ data, int nfields,
+                                                                       Geo g, const cx<R> *__restrict__ tw,
+                                                                       long mstride = 0) {
   constexpr int E = Sched<N>::E, TPP = N / E;
   __shared__ cx<R> lds[T * padlen<N>()];
   const int p = threadIdx.x % T, t = threadIdx.x / T;
   const int k = blockIdx.x * T + p;
-  const bool ok = k < N / 2 + 1;
-  cx<R> *base = data + (long)blockIdx.y * N * g.nfp + k;
-  for (int f = 0; f < nfields; f++) {
-    cx<R> *d = base + f * g.fs;
-    cx<R> v[E];
+  const bool ok = k < (TRUNC ? g.nk : N / 2 + 1);
+  cx<R> *base = data + blockIdx.z * mstride + (long)blockIdx.y * N * g.nfp + k;
+  auto load = [&](cx<R>(&v)[E], int f) {
+    const cx<R> *d = base + f * g.fs;
 #pragma unroll
     for (int c = 0; c < E; c++) v[c] = ok ? d[(long)(t + TPP * c) * g.nfp] : zero<R>();
+  };
+  cx<R> v[E], nx[E];
+  load(v, 0);
+  for (int f = 0; f < nfields; f++) {
+    if (PIPE && f + 1 < nfields) load(nx, f + 1);
     fft_pencil<N, R, INV>(v, t, lds, TileSlot<N, T>{p}, tw);
     if (ok) {
+      cx<R> *d = base + f * g.fs;
 #pragma unroll
-      for (int c = 0; c < E; c++) d[(long)(t + TPP * c) * g.nfp] = v[c];
+      for (int c = 0; c < E; c++)
+        if (!TRUNC || in_box(t + TPP * c, N, g.nk)) st<R>(d + (long)(t + TPP * c) * g.nfp, v[c]);
+    }
+    if (f + 1 < nfields) {
+      if (PIPE) {
+#pragma unroll
+        for (int c = 0; c < E; c++) v[c] = nx[c];
+      } else {
+        load(v, f + 1);
+      }
     }
   }
 }
@@ -89,7 +120,7 @@ __global__ __launch_bounds__(T *(N / Sched<N>::E)) void k_xplain(const cx<RI> *i
     fft_pencil<N, R, INV>(v, t, lds, TileSlot<N, T>{p}, tw);
     if (ok) {
 #pragma unroll
-      for (int c = 0; c < E; c++) out[f * g.fs + off + (t + TPP * c) * istr] = v[c];
+      for (int c = 0; c < E; c++) st<R>(out + f * g.fs + off + (t + TPP * c) * istr, v[c]);
     }
   }
 }
@@ -99,31 +130,22 @@ __global__ __launch_bounds__(T *(N / Sched<N>::E)) void k_xplain(const cx<RI> *i
 // Writes omega partials to W fields 3,4,5 and, if WITH_U, the velocity
 // partials to W fields 0,1,2.  The curl follows main.c:359-361:
 //   curlZ = I*(kx V - ky U), curlY = I*(kz U - kx W), curlX = I*(ky W - kz V)
-// with I*z = (-Im z, Re z).  Grid (ceil(nf/T), n): blockIdx.y = j.
-template <int N, typename R, int T, bool WITH_U, typename SI>
-__global__ __launch_bounds__(T *(N / Sched<N>::E)) void k_xinv_curl(const cx<SI> *__restrict__ uh,
-                                                                     cx<R> *__restrict__ W, Geo g,
-                                                                     const cx<R> *__restrict__ tw) {
+// with I*z = (-Im z, Re z).  Blocks own x-pencil tiles: grid (ceil(nf/T), n),
+// blockIdx.y = j.  U, V, Wv hold the spectral velocity at the thread's
+// positions i = t + TPP*c of pencil (j, k).
+template <int N, typename R, int T, bool WITH_U>
+__device__ __forceinline__ void curl_inverse_x(cx<R> (&U)[Sched<N>::E], cx<R> (&V)[Sched<N>::E],
+                                               cx<R> (&Wv)[Sched<N>::E], cx<R> *__restrict__ W, Geo g, int t,
+                                               int p, int j, int k, bool ok, cx<R> *lds,
+                                               const cx<R> *__restrict__ tw) {
   constexpr int E = Sched<N>::E, TPP = N / E;
-  __shared__ cx<R> lds[T * padlen<N>()];
-  const int p = threadIdx.x % T, t = threadIdx.x / T;
-  const int j = blockIdx.y, k = blockIdx.x * T + p;
-  const bool ok = k < N / 2 + 1;
   const long off = (long)j * g.nfp + k, istr = (long)N * g.nfp;
   const R ky = (R)wave<N>(j), kz = (R)k;
   const TileSlot<N, T> slot{p};
-  cx<R> U[E], V[E], Wv[E];
-#pragma unroll
-  for (int c = 0; c < E; c++) {
-    const long l = off + (t + TPP * c) * istr;
-    U[c] = ok ? cvt<R, SI>(uh[l]) : zero<R>();
-    V[c] = ok ? cvt<R, SI>(uh[g.fs + l]) : zero<R>();
-    Wv[c] = ok ? cvt<R, SI>(uh[2 * g.fs + l]) : zero<R>();
-  }
   auto store = [&](int f, cx<R>(&v)[E]) {
     if (ok) {
 #pragma unroll
-      for (int c = 0; c < E; c++) W[f * g.fs + off + (t + TPP * c) * istr] = v[c];
+      for (int c = 0; c < E; c++) st<R>(W + f * g.fs + off + (t + TPP * c) * istr, v[c]);
     }
   };
   {
@@ -168,6 +190,29 @@ __global__ __launch_bounds__(T *(N / Sched<N>::E)) void k_xinv_curl(const cx<SI>
   }
 }
 
+template <int N, typename R, int T, bool WITH_U, typename SI, int MINW = 1>
+__global__ __launch_bounds__(T *(N / Sched<N>::E), MINW) void k_xinv_curl(const cx<SI> *__restrict__ uh,
+                                                                     cx<R> *__restrict__ W, Geo g,
+                                                                     const cx<R> *__restrict__ tw) {
+  constexpr int E = Sched<N>::E, TPP = N / E;
+  __shared__ cx<R> lds[T * padlen<N>()];
+  const int p = threadIdx.x % T, t = threadIdx.x / T;
+  const int j = blockIdx.y, k = blockIdx.x * T + p;
+  const bool ok = k < N / 2 + 1;
+  const long off = (long)j * g.nfp + k, istr = (long)N * g.nfp;
+  uh += blockIdx.z * g.mS;
+  W += blockIdx.z * g.mW;
+  cx<R> U[E], V[E], Wv[E];
+#pragma unroll
+  for (int c = 0; c < E; c++) {
+    const long l = off + (t + TPP * c) * istr;
+    U[c] = ok ? cvt<R, SI>(uh[l]) : zero<R>();
+    V[c] = ok ? cvt<R, SI>(uh[g.fs + l]) : zero<R>();
+    Wv[c] = ok ? cvt<R, SI>(uh[2 * g.fs + l]) : zero<R>();
+  }
+  curl_inverse_x<N, R, T, WITH_U>(U, V, Wv, W, g, t, p, j, k, ok, lds, tw);
+}
+
 // ---------------------------------------------------------------------------
 // z-pencil helpers.  A block owns PPW contiguous rows (i,j); thread t of a
 // row holds positions pos = t + TPP*c.
@@ -194,7 +239,7 @@ __device__ __forceinline__ void herm_pair(cx<R> (&z)[Sched<N>::E], const cx<R> *
 // A_k = (F_k + conj F_{N-k})/2 and B_k = -i (F_k - conj F_{N-k})/2, k <= N/2.
 template <int N, typename R, class Slot>
 __device__ __forceinline__ void split_store(const cx<R> (&f)[Sched<N>::E], cx<R> *A, cx<R> *B, int t, cx<R> *lds,
-                                            Slot slot) {
+                                            Slot slot, int lim = N / 2 + 1) {
   constexpr int E = Sched<N>::E, TPP = N / E, H = N / 2;
 #pragma unroll
   for (int c = 0; c < E; c++) lds[slot(t + TPP * c)] = f[c];
@@ -202,13 +247,13 @@ __device__ __forceinline__ void split_store(const cx<R> (&f)[Sched<N>::E], cx<R>
 #pragma unroll
   for (int c = 0; c <= E / 2; c++) {
     const int pos = t + TPP * c;
-    if (pos <= H) {
+    if (pos <= H && pos < lim) {
       const cx<R> a = lds[slot(pos)];
       const cx<R> b = lds[slot(pos == 0 ? 0 : N - pos)];
       const R sx = (R)0.5 * (a.x + b.x), sy = (R)0.5 * (a.y - b.y);
       const R dx = (R)0.5 * (a.x - b.x), dy = (R)0.5 * (a.y + b.y);
-      A[pos] = mk<R>(sx, sy);
-      B[pos] = mk<R>(dy, -dx);
+      st<R>(A + pos, mk<R>(sx, sy));
+      st<R>(B + pos, mk<R>(dy, -dx));
     }
   }
   __syncthreads();
@@ -221,15 +266,16 @@ __device__ __forceinline__ void split_store(const cx<R> (&f)[Sched<N>::E], cx<R>
 //  MODE 2 (stage 4): as MODE 1 and the physical u,v,w are written to `us`.
 // Output: N = u x omega (main.c:374-376) transformed along z into W 0..2.
 // Grid n*n/PPW blocks.
-template <int N, typename R, int PPW, int MODE, typename UR>
-__global__ __launch_bounds__(PPW *(N / Sched<N>::E)) void k_zfused(cx<R> *__restrict__ W, UR *__restrict__ us,
+template <int N, typename R, int PPW, int MODE, typename UR, int MINW = 1>
+__global__ __launch_bounds__(PPW *(N / Sched<N>::E), MINW) void k_zfused(cx<R> *__restrict__ W, UR *__restrict__ us,
                                                                     Geo g, R invn3,
                                                                     const cx<R> *__restrict__ tw) {
   constexpr int E = Sched<N>::E, TPP = N / E, H = N / 2;
   __shared__ cx<R> lds[PPW * padlen<N>()];
   const int p = threadIdx.x / TPP, t = threadIdx.x % TPP;
   const long row = (long)blockIdx.x * PPW + p;
-  cx<R> *w = W + row * g.nfp;
+  us += blockIdx.z * g.mR;
+  cx<R> *w = W + blockIdx.z * g.mW + row * g.nfp;
   const PencilSlot<N> slot{p * padlen<N>()};
   R u[E], v[E], ww[E], ox[E], oy[E], oz[E];
   if constexpr (MODE == 0) {
@@ -272,9 +318,9 @@ __global__ __launch_bounds__(PPW *(N / Sched<N>::E)) void k_zfused(cx<R> *__rest
 #pragma unroll
       for (int c = 0; c < E; c++) {
         const long l = row * N + t + TPP * c;
-        us[l] = (UR)u[c];
-        us[g.rs + l] = (UR)v[c];
-        us[2 * g.rs + l] = (UR)ww[c];
+        st_real<UR>(us + l, (UR)u[c]);
+        st_real<UR>(us + g.rs + l, (UR)v[c]);
+        st_real<UR>(us + 2 * g.rs + l, (UR)ww[c]);
       }
     }
   }
@@ -287,12 +333,13 @@ __global__ __launch_bounds__(PPW *(N / Sched<N>::E)) void k_zfused(cx<R> *__rest
   fft_pencil<N, R, false>(f1, t, lds, slot, tw);
   fft_pencil<N, R, false>(f2, t, lds, slot, tw);
   // f2 is the transform of a real pencil: its first half is the r2c output.
+  // Only kz < kmax is kept: the rest is removed by the dealias mask.
 #pragma unroll
   for (int c = 0; c <= E / 2; c++) {
     const int pos = t + TPP * c;
-    if (pos <= H) w[2 * g.fs + pos] = f2[c];
+    if (pos <= H && pos < g.nk) st<R>(w + 2 * g.fs + pos, f2[c]);
   }
-  split_store<N, R>(f1, w, w + g.fs, t, lds, slot);
+  split_store<N, R>(f1, w, w + g.fs, t, lds, slot, g.nk);
 }
 
 // Plain r2c along z of `nfields` physical double fields into spectral
@@ -322,7 +369,7 @@ __global__ __launch_bounds__(PPW *(N / Sched<N>::E)) void k_zr2c(const double *_
 #pragma unroll
       for (int c = 0; c <= E / 2; c++) {
         const int pos = t + TPP * c;
-        if (pos <= H) o[pos] = z[c];
+        if (pos <= H) st<R>(o + pos, z[c]);
       }
     }
   }
@@ -343,7 +390,7 @@ __global__ __launch_bounds__(PPW *(N / Sched<N>::E)) void k_zc2r(const cx<R> *__
   herm_pair<N, R>(z, in + row * g.nfp, nullptr, t);
   fft_pencil<N, R, true>(z, t, lds, slot, tw);
 #pragma unroll
-  for (int c = 0; c < E; c++) out[row * N + t + TPP * c] = (double)(z[c].x * scale);
+  for (int c = 0; c < E; c++) st_real<double>(out + row * N + t + TPP * c, (double)(z[c].x * scale));
 }
 
 // ---------------------------------------------------------------------------
@@ -354,40 +401,58 @@ __global__ __launch_bounds__(PPW *(N / Sched<N>::E)) void k_zc2r(const cx<R> *__
 //  STAGE 1,2: Ur' = Un + b dU;  acc += a dU
 //  STAGE 3  : Un = acc + a dU                     (main.c:400-420)
 // The epilogue runs in double whatever R is.  phat (optional) receives P.
+// The new state (Ur' for stages 0..2, Un for stage 3) is returned in
+// U,V,Wv as the next stage's pass A would read it back from memory.
 // Grid (ceil(nf/T), n): blockIdx.y = j.
 template <int N, typename R, int T, int STAGE, typename SR>
-__global__ __launch_bounds__(T *(N / Sched<N>::E)) void k_xfwd_rk(const cx<R> *__restrict__ W,
-                                                                   cx<SR> *__restrict__ uhr,
-                                                                   double2 *__restrict__ uhn,
-                                                                   double2 *__restrict__ acc,
-                                                                   double2 *__restrict__ phat, Geo g, RKParams q,
-                                                                   const cx<R> *__restrict__ tw) {
+__device__ __forceinline__ void forward_x_rk(const cx<R> *__restrict__ W, cx<SR> *__restrict__ uhr,
+                                             double2 *__restrict__ uhn, double2 *__restrict__ acc,
+                                             double2 *__restrict__ phat, Geo g, RKParams q,
+                                             const cx<R> *__restrict__ tw, int t, int p, int j, int k, bool ok,
+                                             cx<R> *lds, cx<R> (&U)[Sched<N>::E], cx<R> (&V)[Sched<N>::E],
+                                             cx<R> (&Wv)[Sched<N>::E]) {
 #pragma clang fp contract(off)
   constexpr int E = Sched<N>::E, TPP = N / E;
-  __shared__ cx<R> lds[T * padlen<N>()];
-  const int p = threadIdx.x % T, t = threadIdx.x / T;
-  const int j = blockIdx.y, k = blockIdx.x * T + p;
-  const bool ok = k < N / 2 + 1;
   const long off = (long)j * g.nfp + k, istr = (long)N * g.nfp;
   const TileSlot<N, T> slot{p};
+  W += blockIdx.z * g.mW;
+  uhr += blockIdx.z * g.mS;
+  uhn += blockIdx.z * g.mS;
+  acc += blockIdx.z * g.mS;
+  if (phat) phat += blockIdx.z * g.fs;
+  const double2 nd = q.nudt[blockIdx.z];
+  const double nu = nd.x, dt = nd.y;
+  // The y and z passes produced the nonlinear term only inside the
+  // dealiased box (|ky|, kz < kmax); tiles outside it have N = 0 and skip the
+  // transform (block-uniform branch).
+  const bool nl = in_box(j, N, g.nk) && (int)(blockIdx.x * T) < g.nk;
+  const bool okn = k < g.nk;
   cx<R> F[3][E];
+  if (nl) {
 #pragma unroll
-  for (int f = 0; f < 3; f++) {
+    for (int f = 0; f < 3; f++) {
 #pragma unroll
-    for (int c = 0; c < E; c++) F[f][c] = ok ? W[f * g.fs + off + (t + TPP * c) * istr] : zero<R>();
+      for (int c = 0; c < E; c++) F[f][c] = okn ? W[f * g.fs + off + (t + TPP * c) * istr] : zero<R>();
+    }
+#pragma unroll
+    for (int f = 0; f < 3; f++) fft_pencil<N, R, false>(F[f], t, lds, slot, tw);
+  } else {
+#pragma unroll
+    for (int f = 0; f < 3; f++)
+#pragma unroll
+      for (int c = 0; c < E; c++) F[f][c] = zero<R>();
   }
-#pragma unroll
-  for (int f = 0; f < 3; f++) fft_pencil<N, R, false>(F[f], t, lds, slot, tw);
-  if (!ok) return;
   const double ky = (double)wave<N>(j), kz = (double)k;
-  const double nudt = q.nu * q.dt;
+  const double nudt = nu * dt;
   const bool myz = fabs(ky) < q.kmax && fabs(kz) < q.kmax;
 #pragma unroll
   for (int c = 0; c < E; c++) {
+    U[c] = V[c] = Wv[c] = zero<R>();
+    if (!ok) continue;
     const long l = off + (t + TPP * c) * istr;
     const double kx = (double)wave<N>(t + TPP * c);
     const double kk = kx * kx + ky * ky + kz * kz;
-    const double fac = (myz && fabs(kx) < q.kmax) ? q.dt : 0.0;
+    const double fac = (myz && fabs(kx) < q.kmax) ? dt : 0.0;
     double2 d[3];
 #pragma unroll
     for (int f = 0; f < 3; f++) d[f] = make_double2((double)F[f][c].x * fac, (double)F[f][c].y * fac);
@@ -396,9 +461,10 @@ __global__ __launch_bounds__(T *(N / Sched<N>::E)) void k_xfwd_rk(const cx<R> *_
       P.x = (d[0].x * kx + d[1].x * ky + d[2].x * kz) / kk;
       P.y = (d[0].y * kx + d[1].y * ky + d[2].y * kz) / kk;
     }
-    if (phat) phat[l] = P;
+    if (phat) st<double>(phat + l, P);
     const double vk = nudt * kk;
     const double kv[3] = {kx, ky, kz};
+    cx<R> nxt[3];
 #pragma unroll
     for (int f = 0; f < 3; f++) {
       const long lf = f * g.fs + l;
@@ -408,19 +474,66 @@ __global__ __launch_bounds__(T *(N / Sched<N>::E)) void k_xfwd_rk(const cx<R> *_
       d[f].x -= P.x * kv[f] + vk * ur.x;
       d[f].y -= P.y * kv[f] + vk * ur.y;
       if constexpr (STAGE == 0) {
-        uhr[lf] = cvt<SR, double>(make_double2(ur.x + q.b * d[f].x, ur.y + q.b * d[f].y));
-        acc[lf] = make_double2(ur.x + q.a * d[f].x, ur.y + q.a * d[f].y);
+        const cx<SR> nr = cvt<SR, double>(make_double2(ur.x + q.b * d[f].x, ur.y + q.b * d[f].y));
+        st<SR>(uhr + lf, nr);
+        st<double>(acc + lf, make_double2(ur.x + q.a * d[f].x, ur.y + q.a * d[f].y));
+        nxt[f] = cvt<R, SR>(nr);
       } else if constexpr (STAGE < 3) {
         const double2 un = uhn[lf];
-        uhr[lf] = cvt<SR, double>(make_double2(un.x + q.b * d[f].x, un.y + q.b * d[f].y));
+        const cx<SR> nr = cvt<SR, double>(make_double2(un.x + q.b * d[f].x, un.y + q.b * d[f].y));
+        st<SR>(uhr + lf, nr);
         const double2 ac = acc[lf];
-        acc[lf] = make_double2(ac.x + q.a * d[f].x, ac.y + q.a * d[f].y);
+        st<double>(acc + lf, make_double2(ac.x + q.a * d[f].x, ac.y + q.a * d[f].y));
+        nxt[f] = cvt<R, SR>(nr);
       } else {
         const double2 ac = acc[lf];
-        uhn[lf] = make_double2(ac.x + q.a * d[f].x, ac.y + q.a * d[f].y);
+        const double2 nn = make_double2(ac.x + q.a * d[f].x, ac.y + q.a * d[f].y);
+        st<double>(uhn + lf, nn);
+        nxt[f] = cvt<R, double>(nn);
       }
     }
+    U[c] = nxt[0];
+    V[c] = nxt[1];
+    Wv[c] = nxt[2];
   }
+}
+
+template <int N, typename R, int T, int STAGE, typename SR, int MINW = 1>
+__global__ __launch_bounds__(T *(N / Sched<N>::E), MINW) void k_xfwd_rk(const cx<R> *__restrict__ W,
+                                                                   cx<SR> *__restrict__ uhr,
+                                                                   double2 *__restrict__ uhn,
+                                                                   double2 *__restrict__ acc,
+                                                                   double2 *__restrict__ phat, Geo g, RKParams q,
+                                                                   const cx<R> *__restrict__ tw) {
+  constexpr int E = Sched<N>::E;
+  __shared__ cx<R> lds[T * padlen<N>()];
+  const int p = threadIdx.x % T, t = threadIdx.x / T;
+  const int j = blockIdx.y, k = blockIdx.x * T + p;
+  const bool ok = k < N / 2 + 1;
+  cx<R> U[E], V[E], Wv[E];
+  forward_x_rk<N, R, T, STAGE, SR>(W, uhr, uhn, acc, phat, g, q, tw, t, p, j, k, ok, lds, U, V, Wv);
+}
+
+// E of stage STAGE fused with A of the following stage (stage STAGE+1, or
+// stage 1 of the next step after STAGE 3): the new state goes from the RK
+// update straight into the curl prologue and inverse x transform, saving
+// the re-read of the state.  W fields 0..2 (this stage's N) are read before
+// the same pencils' fields 0..5 are overwritten with the next partials.
+template <int N, typename R, int T, int STAGE, typename SR>
+__global__ __launch_bounds__(T *(N / Sched<N>::E)) void k_xfwd_rk_xinv(cx<R> *__restrict__ W,
+                                                                       cx<SR> *__restrict__ uhr,
+                                                                       double2 *__restrict__ uhn,
+                                                                       double2 *__restrict__ acc,
+                                                                       double2 *__restrict__ phat, Geo g,
+                                                                       RKParams q, const cx<R> *__restrict__ tw) {
+  constexpr int E = Sched<N>::E;
+  __shared__ cx<R> lds[T * padlen<N>()];
+  const int p = threadIdx.x % T, t = threadIdx.x / T;
+  const int j = blockIdx.y, k = blockIdx.x * T + p;
+  const bool ok = k < N / 2 + 1;
+  cx<R> U[E], V[E], Wv[E];
+  forward_x_rk<N, R, T, STAGE, SR>(W, uhr, uhn, acc, phat, g, q, tw, t, p, j, k, ok, lds, U, V, Wv);
+  curl_inverse_x<N, R, T, (STAGE < 3)>(U, V, Wv, W + blockIdx.z * g.mW, g, t, p, j, k, ok, lds, tw);
 }
 
 // ---------------------------------------------------------------------------
@@ -440,6 +553,8 @@ __global__ __launch_bounds__(BS) void k_diag(const double2 *__restrict__ uh, int
                                              double *__restrict__ part) {
 #pragma clang fp contract(off)
   const int nf = n / 2 + 1;
+  uh += blockIdx.z * g.mS;
+  part += 4L * blockIdx.z * gridDim.x;
   double es = 0, ec = 0, zs = 0, zc = 0;
   for (long row = blockIdx.x; row < (long)n * n; row += gridDim.x) {
     const int i = (int)(row / n), j = (int)(row % n);
@@ -483,6 +598,8 @@ template <int BS>
 __global__ __launch_bounds__(BS) void k_diag_final(const double *__restrict__ part, int nb,
                                                    double *__restrict__ out) {
 #pragma clang fp contract(off)
+  part += 4L * blockIdx.z * nb;
+  out += 4 * blockIdx.z;
   double es = 0, ec = 0, zs = 0, zc = 0;
   for (int b = threadIdx.x; b < nb; b += BS) {
     two_sum_acc(es, ec, part[4 * b]);

@@ -8,25 +8,44 @@ spectral field and Rr = n^3 reals per physical field:
   pass  stage 1                 stages 2,3                stage 4
   A     r 3Uc(f64) w 3Uc        r 3Uc(SR) w 6Uc           as 2,3
   B     r 3Uc w 3Uc             r 6Uc w 6Uc               as 2,3
-  C     r 3Uc+3Rr w 3Uc         r 6Uc w 3Uc               r 6Uc w 3Uc+3Rr
-  D     r 3Uc w 3Uc             as 1                      as 1
-  E     r 3Uc+3Uc(f64)          r 3Uc+3Uc(SR)+6Uc(f64)    r 3Uc+3Uc(SR)+3Uc(f64)
+  C     r 3Uc+3Rr w 3Zc         r 6Uc w 3Zc               r 6Uc w 3Zc+3Rr
+  D     r 3Zc w 3Yc             as 1                      as 1
+  E     r 3Yc+3Uc(f64)          r 3Yc+3Uc(SR)+6Uc(f64)    r 3Yc+3Uc(SR)+3Uc(f64)
         w 3Uc(SR)+3Uc(f64)      w 3Uc(SR)+3Uc(f64)        w 3Uc(f64)
+
+where Zc = n^2 nk and Yc = n (2nk-1) nk count the dealiased part of the
+nonlinear term (nk = #{m >= 0 : m < kmax}, kmax = 2/3 (n/2+1)).
 
 Partials (A..E outputs) and the stale velocity are in the working precision
 R; the step-start state and the accumulator are always double; SR is the
-stage state's precision (= R).  FP64 total: 198 field-units per step
-(a unit = one spectral or physical field at 8 bytes per real scalar).
+stage state's precision (= R).  FP64 total at n=512: 177 field-units per
+step (a unit = one spectral or physical field at 8 bytes per real scalar).
 """
 
 PEAK_HBM_GBS = 8000.0  # MI355X HBM3E, published peak
-PASSES = ("xinv_curl", "ypass_inv", "zfused", "ypass_fwd", "xfwd_rk")
+PASSES = ("xinv_curl", "ypass_inv", "zfused", "ypass_fwd", "xfwd_rk", "xfwd_rk_xinv")
+
+
+def n_box(n):
+    """Wavenumbers m >= 0 with m < kmax = 2/3 (n/2+1) (main.c:231-238)."""
+    kmax = 2.0 / 3.0 * (n // 2 + 1)
+    m = 0
+    while m <= n // 2 and m < kmax:
+        m += 1
+    return m
 
 
 def slot_bytes(n, precision="fp64"):
-    """{slot name: algorithmic bytes of one launch}; slot = pass.s<stage>."""
+    """{slot name: algorithmic bytes of one launch}; slot = pass.s<stage>.
+
+    The forward transform of the nonlinear term is needed only inside the
+    dealiased box (|kx|,|ky|,kz < kmax); C writes n*n*nk, D reads n*n*nk and
+    writes n*(2nk-1)*nk, E reads n*(2nk-1)*nk complex per field."""
     uc = n * n * (n // 2 + 1)
     rr = n ** 3
+    nk = n_box(n)
+    zc = n * n * nk                # after the z pass, kz < kmax
+    yc = n * (2 * nk - 1) * nk     # after the y pass, also |ky| < kmax
     r = 8 if precision == "fp64" else 4      # real scalar in the working precision
     cr, csr, c64 = 2 * r, 2 * r, 16
     out = {}
@@ -34,21 +53,31 @@ def slot_bytes(n, precision="fp64"):
         a = 3 * uc * c64 + 3 * uc * cr if st == 0 else 3 * uc * csr + 6 * uc * cr
         b = (2 * 3 if st == 0 else 2 * 6) * uc * cr
         if st == 0:
-            c = 3 * uc * cr + 3 * rr * r + 3 * uc * cr
+            c = 3 * uc * cr + 3 * rr * r + 3 * zc * cr
         elif st < 3:
-            c = 6 * uc * cr + 3 * uc * cr
+            c = 6 * uc * cr + 3 * zc * cr
         else:
-            c = 6 * uc * cr + 3 * uc * cr + 3 * rr * r
-        d = 2 * 3 * uc * cr
+            c = 6 * uc * cr + 3 * zc * cr + 3 * rr * r
+        d = 3 * zc * cr + 3 * yc * cr
         if st == 0:
-            e = 3 * uc * cr + 3 * uc * c64 + 3 * uc * csr + 3 * uc * c64
+            e = 3 * yc * cr + 3 * uc * c64 + 3 * uc * csr + 3 * uc * c64
         elif st < 3:
-            e = 3 * uc * cr + 3 * uc * csr + 6 * uc * c64 + 3 * uc * csr + 3 * uc * c64
+            e = 3 * yc * cr + 3 * uc * csr + 6 * uc * c64 + 3 * uc * csr + 3 * uc * c64
         else:
-            e = 3 * uc * cr + 3 * uc * csr + 3 * uc * c64 + 3 * uc * c64
-        for p, v in zip(PASSES, (a, b, c, d, e)):
+            e = 3 * yc * cr + 3 * uc * csr + 3 * uc * c64 + 3 * uc * c64
+        # E fused with the next stage's A: A's writes without its read
+        nxt_writes = 3 * uc * cr if st == 3 else 6 * uc * cr
+        for p, v in zip(PASSES, (a, b, c, d, e, e + nxt_writes)):
             out[f"{p}.s{st + 1}"] = v
     return out
+
+
+def step_bytes_fused(n, precision="fp64"):
+    """Bytes of one step as dns_step runs it in steady state: pass A is fused
+    into the previous stage's E (the standalone A runs only after the state
+    is set)."""
+    sb = slot_bytes(n, precision)
+    return sum(v for k, v in sb.items() if k.split(".")[0] in ("ypass_inv", "zfused", "ypass_fwd", "xfwd_rk_xinv"))
 
 
 def step_bytes(n, precision="fp64"):

@@ -101,14 +101,38 @@ int dns_irfft3(dns_ctx *ctx, const double *spec, double *real);
 
 /* Kernel timing.  When on, every kernel launched by dns_step is bracketed
  * by HIP events on the context's stream; times accumulate per slot
- * 5*stage + pass (stage 0..3, pass 0..4 = x-inverse with curl, y-inverse,
- * fused z pass, y-forward, x-forward with the RK update).  Switching on or
- * off resets the sums.  dns_kernel_timing synchronises and fills
- * ms[DNS_KERNEL_SLOTS] and launches[DNS_KERNEL_SLOTS]. */
-#define DNS_KERNEL_SLOTS 20
+ * 6*stage + pass (stage 0..3; pass 0..5 = x-inverse with curl, y-inverse,
+ * fused z pass, y-forward, x-forward with the RK update, and x-forward with
+ * the RK update fused with the next stage's x-inverse).  Switching on or off
+ * resets the sums.  dns_kernel_timing synchronises and fills
+ * ms[DNS_KERNEL_SLOTS] and launches[DNS_KERNEL_SLOTS]; dns_kernel_slot_name
+ * gives "<pass>.s<stage 1..4>". */
+#define DNS_KERNEL_SLOTS 24
 int dns_set_kernel_timing(dns_ctx *ctx, int on);
 int dns_kernel_timing(dns_ctx *ctx, double *ms, long *launches);
 const char *dns_kernel_slot_name(int slot);
+
+/* Batched ensembles: `members` independent simulations, member m on grid
+ * n[m] (any mix of supported sizes) with viscosity nu[m] and time step
+ * dt[m], all in one precision on one device.  Members of equal n share
+ * every kernel launch (one more grid dimension); the sizes run as
+ * concurrent branches of one HIP graph captured at the first
+ * dns_batch_step and replayed once per step.  Each member follows exactly
+ * the arithmetic of a single context (dns_create) with the same n, nu, dt
+ * and precision.  No reference counterpart (main.c runs one simulation per
+ * process; run.sh:1-10 and hal.sh:1-16 run the Re sweep as separate
+ * processes); these replace such sweeps. */
+typedef struct dns_batch dns_batch;
+int dns_batch_create(dns_batch **out, int members, const int *n, const double *nu, const double *dt,
+                     int precision, int device);
+void dns_batch_destroy(dns_batch *b);
+int dns_batch_load_velocity(dns_batch *b, int member, const double *uvw);
+int dns_batch_step(dns_batch *b, long nsteps);
+int dns_batch_diagnostics(dns_batch *b, int member, double *energy, double *enstrophy);
+int dns_batch_get_spectral(dns_batch *b, int member, double *uvw_hat);
+int dns_batch_get_stale_velocity(dns_batch *b, int member, double *uvw);
+int dns_batch_member_n(dns_batch *b, int member);
+long long dns_batch_device_bytes(const dns_batch *b);
 
 /* Bytes of device memory held by the context. */
 long long dns_device_bytes(const dns_ctx *ctx);
