@@ -118,6 +118,13 @@ def main():
                  "passes": {k: {"avg_ms": v["avg_ms"], "GBs": v["GBs"], "frac": v["frac"]}
                             for k, v in rmx["passes"].items()}}
     dom_name, dom = max(r64["passes"].items(), key=lambda kv: kv[1]["ms"])
+    # HBM traffic per launch measured by rocprofv3 FETCH_SIZE/WRITE_SIZE runs
+    # (calibrated on a copy kernel), committed with the profiles
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_n{n}_fp64.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("traffic_bytes_per_launch", {}).get(dom_name)
     step_b = roofline.step_bytes_fused(n, "fp64")
     line = {
         "metric": METRIC, "value": r64["steps_per_s"], "unit": "steps/s", "n_gpus": 1,
@@ -127,7 +134,7 @@ def main():
         "config": {"workload": f"Taylor-Green vortex n={n}, nu={nu:.6g}, dt={dt:g}, one RK4 step (FP64)",
                    "n": n, "nu": nu, "dt": dt, "precision": "fp64"},
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": dom["GBs"], "peak": roofline.PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": dom["frac"], "traffic": None,
+                     "unit": "GB/s", "frac": dom["frac"], "traffic": traffic,
                      "bytes_per_launch": dom["bytes_per_launch"], "avg_launch_ms": dom["avg_ms"]},
         "step_roofline": {"algorithmic_bytes_per_step": step_b,
                           "achieved_GBs": step_b * r64["steps_per_s"] / 1e9,

@@ -73,8 +73,10 @@ def build_tools(verbose=False):
     ic = os.path.join(BIN, "dns-ic")
     _run([CC, *cflags, os.path.join(PKG, "cli", "dns_ic.c"), f"-L{PKG}", "-ldns_hip",
           "-Wl,-rpath,$ORIGIN/..", "-lm", "-o", ic])
+    probe = os.path.join(BIN, "hbm_probe")
+    _run([HIPCC, *HIP_FLAGS, os.path.join(PKG, "probe", "hbm_probe.hip"), "-o", probe])
     if verbose:
-        print("built", dns, ic)
+        print("built", dns, ic, probe)
     return dns
 
 
