@@ -85,6 +85,89 @@ def cpu_baseline(n, nu, dt, u, steps):
                       f"(FFTW is not installed)"}
 
 
+def iso_field(n, seed):
+    """Seeded isotropic field (E(k) ~ k^4 exp(-2 (k/4)^2), energy 0.5) from dns-ic."""
+    import subprocess
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "iso.raw")
+        subprocess.run([os.path.join(ROOT, "dns_amd", "bin", "dns-ic"), "-t", "iso", "-N", str(n), "-S", str(seed),
+                        "-o", path], check=True, capture_output=True)
+        return np.fromfile(path).reshape(4, n, n, n)[:3].copy()
+
+
+def ensemble(a):
+    """24 simulations, n in {64,96,128,192} x Re in {100,...,3000}, half
+    Taylor-Green and half isotropic, mixed precision, one graph per step."""
+    import dns_amd
+    res = [100, 200, 400, 800, 1600, 3000]
+    ns, nus = [], []
+    for n in (64, 96, 128, 192):
+        for r in res:
+            ns.append(n)
+            nus.append(1.0 / r)
+    dt = 1e-3
+    b = dns_amd.Batch(ns, nus, dt, "mixed")
+    for m, n in enumerate(ns):
+        b.load_velocity(m, tgv(n) if m % 2 == 0 else iso_field(n, 20261202 + m))
+    b.step(a.warmup)
+    b.diagnostics(0)
+    t0 = time.perf_counter()
+    b.step(a.steps)
+    e = [b.diagnostics(m) for m in range(len(ns))]
+    wall = time.perf_counter() - t0
+    pts = sum(n ** 3 for n in ns)
+    line = {"metric": "ensemble RK4 steps/s (24 members, n in {64,96,128,192}, mixed precision)",
+            "value": a.steps / wall, "unit": "ensemble steps/s", "n_gpus": 1, "steps": a.steps,
+            "warmup": a.warmup, "ms_per_step": 1e3 * wall / a.steps, "higher_is_better": True,
+            "member_steps_per_s": len(ns) * a.steps / wall, "grid_points": pts,
+            "points_per_s": pts * a.steps / wall, "dtype": "f32+f64", "data": "synthetic",
+            "config": {"workload": "batched ensemble, one captured graph per step", "members": len(ns),
+                       "n": sorted(set(ns)), "Re": res, "dt": dt},
+            "finite": all(np.isfinite(x[0]) for x in e)}
+    b.close()
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def adjoint(a):
+    """Gradient of the final enstrophy, n=128 isotropic (k0=4, E=0.5),
+    nu=1/800, dt=2e-3, 200 steps, checkpoint every 20; checked against
+    central differences (eps=1e-6) of FP64 runs along 8 seeded
+    divergence-free directions."""
+    import dns_amd
+    n, nu, dt, nsteps, every = 128, 1.0 / 800, 2e-3, a.adj_steps, 20
+    u0 = iso_field(n, 20261202)
+    s = dns_amd.Solver(n, nu, dt, "fp64")
+    t0 = time.perf_counter()
+    J = s.adjoint_forward(u0, nsteps, every)
+    t1 = time.perf_counter()
+    g = s.adjoint_backward()
+    t2 = time.perf_counter()
+    errs = []
+    for k in range(a.adj_dirs):
+        d = iso_field(n, 777 + k)
+        vals = []
+        for sgn in (1, -1):
+            s.load_velocity(u0 + sgn * 1e-6 * d)
+            s.step(nsteps)
+            vals.append(s.diagnostics()[1])
+        fd = (vals[0] - vals[1]) / 2e-6
+        ad = float(np.sum(g * d))
+        errs.append(abs(ad - fd) / abs(fd))
+        print(f"adjoint: direction {k}: adjoint {ad:.12e} fd {fd:.12e} rel {errs[-1]:.2e}", file=sys.stderr,
+              flush=True)
+    s.close()
+    line = {"metric": "adjoint gradient of final enstrophy vs central differences", "value": max(errs),
+            "unit": "max relative error", "higher_is_better": False, "n_gpus": 1, "dtype": "f64",
+            "data": "synthetic", "J": J, "forward_s": t1 - t0, "backward_s": t2 - t1,
+            "backward_over_forward": (t2 - t1) / (t1 - t0), "rel_errors": errs,
+            "config": {"workload": "gradient of final enstrophy w.r.t. initial velocity", "n": n, "nu": nu, "dt": dt,
+                       "steps": nsteps, "checkpoint_every": every, "eps": 1e-6, "directions": a.adj_dirs}}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,11 +177,19 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--ensemble", action="store_true", help="batched ensemble workload instead")
+    ap.add_argument("--adjoint", action="store_true", help="adjoint gradient check instead")
+    ap.add_argument("--adj-steps", type=int, default=200)
+    ap.add_argument("--adj-dirs", type=int, default=8)
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build measures one GPU only; multi-GPU runs are out of scope "
               "(see the sharding design in DESIGN.md)", file=sys.stderr)
         return 2
+    if a.ensemble:
+        return ensemble(a)
+    if a.adjoint:
+        return adjoint(a)
     import dns_amd
     from dns_amd import roofline
 

@@ -62,6 +62,9 @@ def lib():
         "dns_set_kernel_timing": (I, [P, I]),
         "dns_kernel_timing": (I, [P, D, ctypes.POINTER(LL)]),
         "dns_kernel_slot_name": (ctypes.c_char_p, [I]),
+        "dns_adjoint_forward": (I, [P, P, LL, LL, D]),
+        "dns_adjoint_backward": (I, [P, ctypes.c_double, P]),
+        "dns_adjoint_step": (I, [P, P, P, P, P, P, P]),
         "dns_batch_create": (I, [ctypes.POINTER(P), I, P, P, P, I, I]),
         "dns_batch_destroy": (None, [P]),
         "dns_batch_load_velocity": (I, [P, I, P]),
@@ -186,6 +189,32 @@ class Solver:
         a = np.ascontiguousarray(a, dtype=np.float64)
         assert a.shape == (3, self.n, self.n, self.n)
         _check(lib().dns_set_stale_velocity(self._ctx, _ptr(a)), "dns_set_stale_velocity")
+
+    def adjoint_forward(self, u0, nsteps, every=20):
+        """Run nsteps from u0 keeping checkpoints; returns the final enstrophy."""
+        u0 = np.ascontiguousarray(u0, dtype=np.float64)
+        assert u0.shape == (3, self.n, self.n, self.n)
+        z = ctypes.c_double()
+        _check(lib().dns_adjoint_forward(self._ctx, _ptr(u0), int(nsteps), int(every), ctypes.byref(z)),
+               "dns_adjoint_forward")
+        return z.value
+
+    def adjoint_backward(self, seed=1.0):
+        """seed * d(final enstrophy)/d(u0) after adjoint_forward."""
+        g = np.empty((3, self.n, self.n, self.n), dtype=np.float64)
+        _check(lib().dns_adjoint_backward(self._ctx, float(seed), _ptr(g)), "dns_adjoint_backward")
+        return g
+
+    def adjoint_step(self, uh, us, lam_uh, lam_us):
+        """Adjoint of one step at state (uh, us) applied to (lam_uh, lam_us)."""
+        n, nf = self.n, self.nf
+        a = [np.ascontiguousarray(x, dtype=t) for x, t in
+             ((uh, np.complex128), (us, np.float64), (lam_uh, np.complex128), (lam_us, np.float64))]
+        assert a[0].shape == a[2].shape == (3, n, n, nf) and a[1].shape == a[3].shape == (3, n, n, n)
+        o1 = np.empty((3, n, n, nf), dtype=np.complex128)
+        o2 = np.empty((3, n, n, n), dtype=np.float64)
+        _check(lib().dns_adjoint_step(self._ctx, *[_ptr(x) for x in a], _ptr(o1), _ptr(o2)), "dns_adjoint_step")
+        return o1, o2
 
     def rfft3(self, real):
         real = np.ascontiguousarray(real, dtype=np.float64)

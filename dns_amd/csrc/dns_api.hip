@@ -231,6 +231,7 @@ void dns_destroy(dns_ctx *c) {
   }
   for (hipEvent_t e : c->ev_pool)
     if (e) hipEventDestroy(e);
+  delete (AdjWork *)c->adj;
   if (c->stream) hipStreamDestroy(c->stream);
   delete c->impl;
   delete c;
@@ -343,6 +344,33 @@ const char *dns_kernel_slot_name(int slot) {
   if (!names[slot][0])
     snprintf(names[slot], sizeof names[slot], "%s.s%d", pass[slot % DNS_NPASS], slot / DNS_NPASS + 1);
   return names[slot];
+}
+
+int dns_adjoint_forward(dns_ctx *c, const double *u0, long nsteps, long every, double *enstrophy) {
+  if (!c || !u0 || !enstrophy || nsteps < 1 || every < 1) return DNS_EINVAL;
+  if (c->prec != DNS_FP64 || c->members != 1) return DNS_EINVAL;
+  DNS_HIP(hipSetDevice(c->device));
+  c->have_phat = 0;
+  int rc = c->impl->adj_forward(c, u0, nsteps, every);
+  if (rc) return rc;
+  double e;
+  return ctx_diagnostics(c, 0, &e, enstrophy);
+}
+
+int dns_adjoint_backward(dns_ctx *c, double seed, double *grad) {
+  if (!c || !grad) return DNS_EINVAL;
+  if (c->prec != DNS_FP64 || c->members != 1) return DNS_EINVAL;
+  DNS_HIP(hipSetDevice(c->device));
+  return c->impl->adj_backward(c, seed, grad);
+}
+
+int dns_adjoint_step(dns_ctx *c, const double *uvw_hat, const double *uvw_stale, const double *lam_hat,
+                     const double *lam_stale, double *lam_hat_out, double *lam_stale_out) {
+  if (!c || !uvw_hat || !uvw_stale || !lam_hat || !lam_stale || !lam_hat_out || !lam_stale_out)
+    return DNS_EINVAL;
+  if (c->prec != DNS_FP64 || c->members != 1) return DNS_EINVAL;
+  DNS_HIP(hipSetDevice(c->device));
+  return c->impl->adj_step(c, uvw_hat, uvw_stale, lam_hat, lam_stale, lam_hat_out, lam_stale_out);
 }
 
 int dns_rfft3(dns_ctx *c, const double *real, double *spec) {

@@ -67,6 +67,10 @@ struct dns_ctx {
   int w_ready;
   long long bytes;
   dnsk::Impl *impl;
+  // when set, step() copies the stage state U_{r+1} after stage r (r < 3)
+  // to rec[r] (used by the adjoint's forward recomputation)
+  double2 *rec[3];
+  void *adj;  // adjoint workspace (dnsk::AdjWork), allocated on first use
   // HIP-event timing of the step's kernels on `stream` (off by default)
   int timing;
   std::vector<hipEvent_t> ev_pool;
@@ -97,6 +101,11 @@ struct Impl {
   virtual int irfft3(dns_ctx *c, const double *in, double *out) = 0;
   virtual int get_us(dns_ctx *c, int m, double *out) = 0;
   virtual int set_us(dns_ctx *c, int m, const double *in) = 0;
+  // discrete adjoint (FP64 contexts, member 0)
+  virtual int adj_forward(dns_ctx *c, const double *u0, long nsteps, long every) = 0;
+  virtual int adj_backward(dns_ctx *c, double seed, double *grad) = 0;
+  virtual int adj_step(dns_ctx *c, const double *uh, const double *us, const double *lam_uh,
+                       const double *lam_us, double *out_uh, double *out_us) = 0;
 };
 
 // Event bracketing of one launch when timing is on.
@@ -155,6 +164,10 @@ template <int N> std::vector<long double> twiddles_ld() {
   return t;
 }
 
+}  // namespace dnsk
+#include "adjoint_engine.hpp"
+namespace dnsk {
+
 template <int N, typename R, typename SR> struct Engine final : Impl {
   static constexpr int E = Sched<N>::E, TPP = N / E;
   // Column-tile widths: 128-byte row segments, at most 1024 threads.
@@ -210,7 +223,7 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
     hipStream_t s = c->stream;
     double *tmp = (double *)wslice(c, m);  // 3 physical double fields fit in a member's W
     const size_t rb = sizeof(double) * c->g.rs;
-    DNS_HIP(hipMemcpyAsync(tmp, uvw, 3 * rb, hipMemcpyHostToDevice, s));
+    DNS_HIP(hipMemcpyAsync(tmp, uvw, 3 * rb, hipMemcpyDefault, s));
     int rc = fwd3_double(c, tmp, (double2 *)c->uhn + m * c->g.mS, 3);
     if (rc) return rc;
     R *us = (R *)c->us + m * c->g.mR;
@@ -282,6 +295,8 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
             DNS_TIMED(c, sl + 5, (k_xfwd_rk_xinv<N, R, T_R, 3, SR><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, ph, g, q, tw)));
             break;
         }
+        if (rk < 3 && c->rec[rk])
+          DNS_HIP(hipMemcpyAsync(c->rec[rk], uhr, 3 * sizeof(cx<SR>) * g.fs, hipMemcpyDeviceToDevice, s));
       }
       if (ph) c->have_phat = 1;
     }
@@ -387,6 +402,11 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
     return DNS_OK;
   }
 
+  int adj_forward(dns_ctx *c, const double *u0, long nsteps, long every) override;
+  int adj_backward(dns_ctx *c, double seed, double *grad) override;
+  int adj_step(dns_ctx *c, const double *uh, const double *us, const double *lam_uh, const double *lam_us,
+               double *out_uh, double *out_us) override;
+
   int set_us(dns_ctx *c, int m, const double *in) override {
     c->w_ready = 0;
     hipStream_t s = c->stream;
@@ -404,6 +424,60 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
     return DNS_OK;
   }
 };
+
+template <int N, typename R, typename SR>
+int Engine<N, R, SR>::adj_forward(dns_ctx *c, const double *u0, long nsteps, long every) {
+  if constexpr (sizeof(R) != 8) {
+    return DNS_EINVAL;
+  } else {
+    return Adjoint<N>::forward(c, u0, nsteps, every, nullptr);
+  }
+}
+
+template <int N, typename R, typename SR> int Engine<N, R, SR>::adj_backward(dns_ctx *c, double seed, double *grad) {
+  if constexpr (sizeof(R) != 8) {
+    return DNS_EINVAL;
+  } else {
+    return Adjoint<N>::backward(c, seed, grad);
+  }
+}
+
+template <int N, typename R, typename SR>
+int Engine<N, R, SR>::adj_step(dns_ctx *c, const double *uh, const double *us, const double *lam_uh,
+                               const double *lam_us, double *out_uh, double *out_us) {
+  if constexpr (sizeof(R) != 8) {
+    return DNS_EINVAL;
+  } else {
+    AdjWork *w;
+    int rc = Adjoint<N>::workspace(c, w);
+    if (rc) return rc;
+    hipStream_t s = c->stream;
+    const size_t pitch = sizeof(double2) * c->nfp, width = sizeof(double2) * NF, rows = (size_t)3 * N * N;
+    // step-start state in w->scr / w->lu (scratch until backward_step reuses
+    // them: it copies both into the context first)
+    double2 *U = w->Fw;
+    double *u = w->lw;
+    DNS_HIP(hipMemset2DAsync(U, pitch, 0, pitch, rows, s));
+    DNS_HIP(hipMemcpy2DAsync(U, pitch, uh, width, width, rows, hipMemcpyDefault, s));
+    DNS_HIP(hipMemcpyAsync(u, us, w->R3, hipMemcpyDefault, s));
+    DNS_HIP(hipMemset2DAsync(w->lamU, pitch, 0, pitch, rows, s));
+    DNS_HIP(hipMemcpy2DAsync(w->lamU, pitch, lam_uh, width, width, rows, hipMemcpyDefault, s));
+    DNS_HIP(hipMemcpyAsync(w->lamS, lam_us, w->R3, hipMemcpyDefault, s));
+    // backward_step overwrites Fw and lw; keep the inputs in the segment store
+    void *st = w->seg.empty() ? Adjoint<N>::state_alloc(c, w) : w->seg[0];
+    if (!st) return DNS_ENOMEM;
+    if (w->seg.empty()) w->seg.push_back(st);
+    DNS_HIP(hipMemcpyAsync(st, U, w->C3, hipMemcpyDeviceToDevice, s));
+    DNS_HIP(hipMemcpyAsync((char *)st + w->C3, u, w->R3, hipMemcpyDeviceToDevice, s));
+    rc = Adjoint<N>::backward_step(c, w, (const double2 *)st, (const double *)((char *)st + w->C3));
+    if (rc) return rc;
+    DNS_HIP(hipMemcpy2DAsync(out_uh, width, w->lamU, pitch, width, rows, hipMemcpyDefault, s));
+    DNS_HIP(hipMemcpyAsync(out_us, w->lamS, w->R3, hipMemcpyDefault, s));
+    DNS_HIP(hipStreamSynchronize(s));
+    c->w_ready = 0;
+    return DNS_OK;
+  }
+}
 
 template <int N> Impl *make_engine(int prec) {
   if (prec == DNS_MIXED) return new Engine<N, float, float>();

@@ -112,6 +112,25 @@ int dns_set_kernel_timing(dns_ctx *ctx, int on);
 int dns_kernel_timing(dns_ctx *ctx, double *ms, long *launches);
 const char *dns_kernel_slot_name(int slot);
 
+/* Discrete adjoint (FP64 contexts).  J = final enstrophy sum as printed
+ * by dns (main.c:263-266) after nsteps RK4 steps from the initial velocity
+ * u0 ([3][n][n][n], loaded as dns_load_velocity does).
+ * dns_adjoint_forward runs the steps, keeping (spectral state, stale
+ * velocity) checkpoints every `every` steps on the device, and returns J.
+ * dns_adjoint_backward then returns grad = seed * dJ/du0 by the exact
+ * transpose of the discrete step (adjoint FFTs with the half-spectrum
+ * weights, adjoint curl, projection, viscous term and u x omega, through the
+ * stale-velocity coupling of stage 1), recomputing each segment from its
+ * checkpoint.  After the sweep the context holds an unspecified state.
+ * dns_adjoint_step applies the adjoint of one step at the state
+ * (uvw_hat [3][n][n][n/2+1] complex, uvw_stale [3][n][n][n]) to
+ * (lam_hat, lam_stale) in the same layouts.  Pointers may be host or
+ * device memory.  No reference counterpart (main.c has no derivative). */
+int dns_adjoint_forward(dns_ctx *ctx, const double *u0, long nsteps, long every, double *enstrophy);
+int dns_adjoint_backward(dns_ctx *ctx, double seed, double *grad);
+int dns_adjoint_step(dns_ctx *ctx, const double *uvw_hat, const double *uvw_stale, const double *lam_hat,
+                     const double *lam_stale, double *lam_hat_out, double *lam_stale_out);
+
 /* Batched ensembles: `members` independent simulations, member m on grid
  * n[m] (any mix of supported sizes) with viscosity nu[m] and time step
  * dt[m], all in one precision on one device.  Members of equal n share
