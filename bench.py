@@ -168,6 +168,59 @@ def adjoint(a):
     return 0
 
 
+def validate(a):
+    """Taylor-Green n=256, Re in {100,...,3000}, dt=0.0025, 4000 steps (t=10),
+    FP64, run as one batch and compared line by line with the reference's
+    own trajectories (directory 0256 of the reference, excerpted in
+    tests/golden/tgv_trajectories.json); also the dissipation rate
+    eps = 2 Z / Re (tgv.gp:9-11) and its peak."""
+    import dns_amd
+    with open(os.path.join(ROOT, "tests", "golden", "tgv_trajectories.json")) as f:
+        gold = json.load(f)
+    res = [100, 200, 400, 800, 1600, 3000]
+    n, dt = 256, 0.0025
+    keys = [f"256/{r}" for r in res]
+    nus = [float(gold[k]["nu"]) for k in keys]
+    rows = [{int(r[0]): (float(r[2]), float(r[3])) for r in gold[k]["rows"]} for k in keys]
+    b = dns_amd.Batch([n] * len(res), nus, dt, "fp64")
+    u = tgv(n)
+    for m in range(len(res)):
+        b.load_velocity(m, u)
+    out = {r: {"max_rel_E": 0.0, "max_rel_Z": 0.0, "first_step_over_1e-12": None, "peak_eps": 0.0,
+               "peak_eps_t": 0.0, "gold_peak_eps": 0.0} for r in res}
+    t0 = time.perf_counter()
+    step = 0
+    while step <= a.val_steps:
+        for m, r in enumerate(res):
+            e, z = b.diagnostics(m)
+            o = out[r]
+            if step in rows[m]:
+                ge, gz = rows[m][step]
+                de, dz = abs(e - ge) / ge, abs(z - gz) / gz
+                o["max_rel_E"] = max(o["max_rel_E"], de)
+                o["max_rel_Z"] = max(o["max_rel_Z"], dz)
+                if o["first_step_over_1e-12"] is None and max(de, dz) > 1e-12:
+                    o["first_step_over_1e-12"] = step
+                if 2 * gz / r > o["gold_peak_eps"]:
+                    o["gold_peak_eps"] = 2 * gz / r
+            if 2 * z / r > o["peak_eps"]:
+                o["peak_eps"], o["peak_eps_t"] = 2 * z / r, step * dt
+        if step == a.val_steps:
+            break
+        b.step(10)
+        step += 10
+        if step % 500 == 0:
+            print(f"validate: step {step}", file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    b.close()
+    line = {"metric": "Taylor-Green n=256 trajectories vs the reference's golden runs", "value":
+            max(max(o["max_rel_E"], o["max_rel_Z"]) for o in out.values()), "unit": "max relative deviation",
+            "higher_is_better": False, "n_gpus": 1, "dtype": "f64", "steps": a.val_steps, "wall_s": wall,
+            "per_Re": out, "config": {"workload": "TGV n=256, dt=0.0025, nu=1/Re", "Re": res}}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -181,6 +234,8 @@ def main():
     ap.add_argument("--adjoint", action="store_true", help="adjoint gradient check instead")
     ap.add_argument("--adj-steps", type=int, default=200)
     ap.add_argument("--adj-dirs", type=int, default=8)
+    ap.add_argument("--validate", action="store_true", help="n=256 golden-trajectory validation instead")
+    ap.add_argument("--val-steps", type=int, default=4000)
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build measures one GPU only; multi-GPU runs are out of scope "
@@ -190,6 +245,8 @@ def main():
         return ensemble(a)
     if a.adjoint:
         return adjoint(a)
+    if a.validate:
+        return validate(a)
     import dns_amd
     from dns_amd import roofline
 

@@ -266,7 +266,9 @@ __device__ __forceinline__ void split_store(const cx<R> (&f)[Sched<N>::E], cx<R>
 //  MODE 2 (stage 4): as MODE 1 and the physical u,v,w are written to `us`.
 // Output: N = u x omega (main.c:374-376) transformed along z into W 0..2.
 // Grid n*n/PPW blocks.
-template <int N, typename R, int PPW, int MODE, typename UR, int MINW = 1>
+// PHASED: load and transform one Hermitian pair at a time (lower register
+// pressure, loads not hoisted across the transforms' barriers).
+template <int N, typename R, int PPW, int MODE, typename UR, int MINW = 1, bool PHASED = false>
 __global__ __launch_bounds__(PPW *(N / Sched<N>::E), MINW) void k_zfused(cx<R> *__restrict__ W, UR *__restrict__ us,
                                                                     Geo g, R invn3,
                                                                     const cx<R> *__restrict__ tw) {
@@ -288,8 +290,9 @@ __global__ __launch_bounds__(PPW *(N / Sched<N>::E), MINW) void k_zfused(cx<R> *
       ww[c] = (R)us[2 * g.rs + l];
     }
     herm_pair<N, R>(z1, w + 3 * g.fs, w + 4 * g.fs, t);
-    herm_pair<N, R>(z2, w + 5 * g.fs, nullptr, t);
+    if (!PHASED) herm_pair<N, R>(z2, w + 5 * g.fs, nullptr, t);
     fft_pencil<N, R, true>(z1, t, lds, slot, tw);
+    if (PHASED) herm_pair<N, R>(z2, w + 5 * g.fs, nullptr, t);
     fft_pencil<N, R, true>(z2, t, lds, slot, tw);
 #pragma unroll
     for (int c = 0; c < E; c++) {
@@ -300,10 +303,14 @@ __global__ __launch_bounds__(PPW *(N / Sched<N>::E), MINW) void k_zfused(cx<R> *
   } else {
     cx<R> z1[E], z2[E], z3[E];
     herm_pair<N, R>(z1, w, w + g.fs, t);
-    herm_pair<N, R>(z2, w + 2 * g.fs, w + 3 * g.fs, t);
-    herm_pair<N, R>(z3, w + 4 * g.fs, w + 5 * g.fs, t);
+    if (!PHASED) {
+      herm_pair<N, R>(z2, w + 2 * g.fs, w + 3 * g.fs, t);
+      herm_pair<N, R>(z3, w + 4 * g.fs, w + 5 * g.fs, t);
+    }
     fft_pencil<N, R, true>(z1, t, lds, slot, tw);
+    if (PHASED) herm_pair<N, R>(z2, w + 2 * g.fs, w + 3 * g.fs, t);
     fft_pencil<N, R, true>(z2, t, lds, slot, tw);
+    if (PHASED) herm_pair<N, R>(z3, w + 4 * g.fs, w + 5 * g.fs, t);
     fft_pencil<N, R, true>(z3, t, lds, slot, tw);
 #pragma unroll
     for (int c = 0; c < E; c++) {
