@@ -65,6 +65,12 @@ def lib():
         "dns_adjoint_forward": (I, [P, P, LL, LL, D]),
         "dns_adjoint_backward": (I, [P, ctypes.c_double, P]),
         "dns_adjoint_step": (I, [P, P, P, P, P, P, P]),
+        "dns_launch_shapes": (I, [P]),
+        "dns_set_launch_shape": (I, [P, I]),
+        "dns_get_launch_shape": (I, [P]),
+        "dns_autotune": (I, [P, I, ctypes.POINTER(I)]),
+        "dns_batch_autotune": (I, [P, I]),
+        "dns_batch_launch_shape": (I, [P, I]),
         "dns_batch_create": (I, [ctypes.POINTER(P), I, P, P, P, I, I]),
         "dns_batch_destroy": (None, [P]),
         "dns_batch_load_velocity": (I, [P, I, P]),
@@ -146,6 +152,20 @@ class Solver:
         cnt = (ctypes.c_long * KERNEL_SLOTS)()
         _check(lib().dns_kernel_timing(self._ctx, ms, cnt), "dns_kernel_timing")
         return {lib().dns_kernel_slot_name(i).decode(): (ms[i], cnt[i]) for i in range(KERNEL_SLOTS)}
+
+    def launch_shapes(self):
+        return lib().dns_launch_shapes(self._ctx)
+
+    def set_launch_shape(self, shape):
+        _check(lib().dns_set_launch_shape(self._ctx, int(shape)), "dns_set_launch_shape")
+
+    def launch_shape(self):
+        return lib().dns_get_launch_shape(self._ctx)
+
+    def autotune(self, steps=3):
+        best = ctypes.c_int()
+        _check(lib().dns_autotune(self._ctx, int(steps), ctypes.byref(best)), "dns_autotune")
+        return best.value
 
     def load_velocity(self, uvw):
         uvw = np.ascontiguousarray(uvw, dtype=np.float64)
@@ -277,6 +297,10 @@ class Batch:
 
     def step(self, nsteps=1):
         _check(lib().dns_batch_step(self._b, int(nsteps)), "dns_batch_step")
+
+    def autotune(self, steps=3):
+        _check(lib().dns_batch_autotune(self._b, int(steps)), "dns_batch_autotune")
+        return [lib().dns_batch_launch_shape(self._b, m) for m in range(len(self.ns))]
 
     def diagnostics(self, m):
         e, z = ctypes.c_double(), ctypes.c_double()

@@ -183,6 +183,60 @@ static int ctx_set_spectral(dns_ctx *c, int m, const double *uvw_hat) {
   return DNS_OK;
 }
 
+// Time every launch shape of the context for `steps` steps from its
+// current state, keep the fastest, and restore the state.  All shapes give
+// bitwise identical results.
+static int ctx_autotune(dns_ctx *c, int steps, int *best_out, double *best_ms) {
+  DNS_HIP(hipSetDevice(c->device));
+  const int nv = c->impl->variants();
+  int best = c->variant;
+  double bms = 0;
+  if (nv > 1) {
+    const size_t sb = (size_t)c->members * c->g.mS * sizeof(double2);
+    const size_t ub = (size_t)c->members * c->g.mR * c->impl->r_size();
+    void *su = nullptr, *sus = nullptr;
+    DNS_HIP(hipMalloc(&su, sb));
+    if (hipMalloc(&sus, ub) != hipSuccess) {
+      hipFree(su);
+      return DNS_ENOMEM;
+    }
+    hipEvent_t a, b;
+    hipEventCreate(&a);
+    hipEventCreate(&b);
+    int rc = DNS_OK;
+    hipMemcpyAsync(su, c->uhn, sb, hipMemcpyDeviceToDevice, c->stream);
+    hipMemcpyAsync(sus, c->us, ub, hipMemcpyDeviceToDevice, c->stream);
+    for (int v = 0; v < nv && rc == DNS_OK; v++) {
+      c->impl->set_variant(c, v);
+      rc = c->impl->step(c, 1);
+      hipEventRecord(a, c->stream);
+      if (rc == DNS_OK) rc = c->impl->step(c, steps);
+      hipEventRecord(b, c->stream);
+      hipEventSynchronize(b);
+      float ms = 0;
+      hipEventElapsedTime(&ms, a, b);
+      if (v == 0 || ms < bms) {
+        bms = ms;
+        best = v;
+      }
+    }
+    hipMemcpyAsync(c->uhn, su, sb, hipMemcpyDeviceToDevice, c->stream);
+    hipMemcpyAsync(c->us, sus, ub, hipMemcpyDeviceToDevice, c->stream);
+    hipStreamSynchronize(c->stream);
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    hipFree(su);
+    hipFree(sus);
+    c->w_ready = 0;
+    c->have_phat = 0;
+    if (rc) return rc;
+    c->impl->set_variant(c, best);
+  }
+  if (best_out) *best_out = best;
+  if (best_ms) *best_ms = bms;
+  return DNS_OK;
+}
+
 // A batch: one context per distinct grid size, members mapped to
 // (context, index).  Steps are replayed from one captured graph that runs the
 // contexts on concurrent branches.
@@ -215,7 +269,9 @@ const char *dns_last_error(void) { return g_err; }
 
 int dns_create(dns_ctx **out, int n, double nu, double dt, int precision, int device) {
   if (!out) return DNS_EINVAL;
-  return ctx_create(out, n, 1, &nu, &dt, precision, device);
+  int rc = ctx_create(out, n, 1, &nu, &dt, precision, device);
+  if (rc == DNS_OK) (*out)->use_graph = 1;
+  return rc;
 }
 
 void dns_destroy(dns_ctx *c) {
@@ -232,6 +288,7 @@ void dns_destroy(dns_ctx *c) {
   for (hipEvent_t e : c->ev_pool)
     if (e) hipEventDestroy(e);
   delete (AdjWork *)c->adj;
+  if (c->gexec) hipGraphExecDestroy(c->gexec);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c->impl;
   delete c;
@@ -373,6 +430,20 @@ int dns_adjoint_step(dns_ctx *c, const double *uvw_hat, const double *uvw_stale,
   return c->impl->adj_step(c, uvw_hat, uvw_stale, lam_hat, lam_stale, lam_hat_out, lam_stale_out);
 }
 
+int dns_launch_shapes(dns_ctx *c) { return c ? c->impl->variants() : DNS_EINVAL; }
+
+int dns_set_launch_shape(dns_ctx *c, int shape) {
+  if (!c) return DNS_EINVAL;
+  return c->impl->set_variant(c, shape);
+}
+
+int dns_get_launch_shape(dns_ctx *c) { return c ? c->variant : DNS_EINVAL; }
+
+int dns_autotune(dns_ctx *c, int steps, int *best_shape) {
+  if (!c || steps < 1) return DNS_EINVAL;
+  return ctx_autotune(c, steps, best_shape, nullptr);
+}
+
 int dns_rfft3(dns_ctx *c, const double *real, double *spec) {
   if (!c || !real || !spec) return DNS_EINVAL;
   DNS_HIP(hipSetDevice(c->device));
@@ -497,6 +568,29 @@ int dns_batch_step(dns_batch *b, long nsteps) {
   DNS_HIP(hipEventRecord(b->fork, b->main));
   for (dns_ctx *c : b->groups) DNS_HIP(hipStreamWaitEvent(c->stream, b->fork, 0));
   return DNS_OK;
+}
+
+int dns_batch_autotune(dns_batch *b, int steps) {
+  if (!b || steps < 1) return DNS_EINVAL;
+  for (dns_ctx *c : b->groups) {
+    int rc = ctx_autotune(c, steps, nullptr, nullptr);
+    if (rc) return rc;
+  }
+  if (b->exec) {  // recapture with the chosen shapes
+    hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+  }
+  if (b->graph) {
+    hipGraphDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  return DNS_OK;
+}
+
+int dns_batch_launch_shape(dns_batch *b, int member) {
+  dns_ctx *c;
+  int i, rc = batch_member(b, member, &c, &i);
+  return rc ? rc : c->variant;
 }
 
 int dns_batch_diagnostics(dns_batch *b, int member, double *energy, double *enstrophy) {

@@ -71,6 +71,10 @@ struct dns_ctx {
   // to rec[r] (used by the adjoint's forward recomputation)
   double2 *rec[3];
   void *adj;  // adjoint workspace (dnsk::AdjWork), allocated on first use
+  int variant;            // launch shape (see Engine::step_launch)
+  int use_graph;          // replay steps from a captured graph
+  hipGraphExec_t gexec;   // graph of one step for `gvariant`
+  int gvariant;
   // HIP-event timing of the step's kernels on `stream` (off by default)
   int timing;
   std::vector<hipEvent_t> ev_pool;
@@ -96,6 +100,8 @@ struct Impl {
   // nsteps steps of all members; with `ready` the partials of stage 1 are
   // taken as present (graph capture after prepare()).
   virtual int step(dns_ctx *c, long nsteps) = 0;
+  virtual int variants() const = 0;
+  virtual int set_variant(dns_ctx *c, int v) = 0;
   virtual int snapshot(dns_ctx *c, int m, double *uvwp) = 0;
   virtual int rfft3(dns_ctx *c, const double *in, double *out) = 0;
   virtual int irfft3(dns_ctx *c, const double *in, double *out) = 0;
@@ -246,7 +252,66 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
     return DNS_OK;
   }
 
+  int set_variant(dns_ctx *c, int v) override {
+    if (v < 0 || v >= variants()) return DNS_EINVAL;
+    c->variant = v;
+    return DNS_OK;
+  }
+
+  // Launch shapes selectable at run time (all give identical arithmetic):
+  // bit 0 halves the strided passes' column tile, bit 1 doubles the z pass's
+  // pencils per block.  Only grids up to 256 carry the alternatives.
+  static constexpr bool TUNABLE = N <= 256;
+  static constexpr int T_H = T_R >= 4 ? T_R / 2 : T_R;
+  static constexpr int PPW2 = 2 * PPW;
+  int variants() const override { return TUNABLE ? 4 : 1; }
+
   int step(dns_ctx *c, long nsteps) override {
+    int rc = prepare(c);
+    if (rc) return rc;
+    if (c->use_graph && !c->timing && !c->capture && !c->rec[0]) return step_graph(c, nsteps);
+    return step_launch(c, nsteps);
+  }
+
+  int step_launch(dns_ctx *c, long nsteps) {
+    if constexpr (TUNABLE) {
+      switch (c->variant) {
+        case 1: return step_body<T_H, PPW>(c, nsteps);
+        case 2: return step_body<T_R, PPW2>(c, nsteps);
+        case 3: return step_body<T_H, PPW2>(c, nsteps);
+        default: break;
+      }
+    }
+    return step_body<T_R, PPW>(c, nsteps);
+  }
+
+  // One step captured once into a graph (per launch shape) and replayed.
+  int step_graph(dns_ctx *c, long nsteps) {
+    if (c->gexec && c->gvariant != c->variant) {
+      hipGraphExecDestroy(c->gexec);
+      c->gexec = nullptr;
+    }
+    if (!c->gexec) {
+      hipGraph_t graph = nullptr;
+      DNS_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      int rc = step_launch(c, 1);
+      hipError_t e = hipStreamEndCapture(c->stream, &graph);
+      if (rc || e != hipSuccess) {
+        if (graph) hipGraphDestroy(graph);
+        if (rc) return rc;
+        DNS_HIP(e);
+      }
+      e = hipGraphInstantiate(&c->gexec, graph, nullptr, nullptr, 0);
+      hipGraphDestroy(graph);
+      DNS_HIP(e);
+      c->gvariant = c->variant;
+    }
+    for (long i = 0; i < nsteps; i++) DNS_HIP(hipGraphLaunch(c->gexec, c->stream));
+    c->w_ready = 1;
+    return DNS_OK;
+  }
+
+  template <int TT, int PP> int step_body(dns_ctx *c, long nsteps) {
     hipStream_t s = c->stream;
     const Geo g = c->g;
     const int M = c->members;
@@ -257,11 +322,9 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
     const R invn3 = (R)c->invn3;
     static const double a_rk[4] = {1 / 6.0, 1 / 3.0, 1 / 3.0, 1 / 6.0};  // main.c:15
     static const double b_rk[4] = {0.5, 0.5, 1.0, 0.0};                 // main.c:16
-    const dim3 gx = grid_x(T_R, M), bx(T_R * TPP);
-    const dim3 gz(N * N / PPW, 1, M), bz(PPW * TPP);
-    const dim3 gy_trunc((g.nk + T_R - 1) / T_R, N, M);  // forward y pass: dealiased kz columns only
-    int rc = prepare(c);
-    if (rc) return rc;
+    const dim3 gx = grid_x(TT, M), bx(TT * TPP);
+    const dim3 gz(N * N / PP, 1, M), bz(PP * TPP);
+    const dim3 gy_trunc((g.nk + TT - 1) / TT, N, M);  // forward y pass: dealiased kz columns only
     for (long st = 0; st < nsteps; st++) {
       double2 *ph = (c->capture && st == nsteps - 1) ? (double2 *)c->phat : nullptr;
       for (int rk = 0; rk < 4; rk++) {
@@ -269,30 +332,30 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
         const int sl = DNS_NPASS * rk;
         // pass A of this stage ran fused into the previous stage's E
         if (rk == 0) {
-          DNS_TIMED(c, sl + 1, (k_ypass<N, R, T_R, true><<<gx, bx, 0, s>>>(W + 3 * g.fs, 3, g, tw, g.mW)));
-          DNS_TIMED(c, sl + 2, (k_zfused<N, R, PPW, 0, R, 1, true><<<gz, bz, 0, s>>>(W, (R *)c->us, g, invn3, tw)));
+          DNS_TIMED(c, sl + 1, (k_ypass<N, R, TT, true><<<gx, bx, 0, s>>>(W + 3 * g.fs, 3, g, tw, g.mW)));
+          DNS_TIMED(c, sl + 2, (k_zfused<N, R, PP, 0, R, 1, true><<<gz, bz, 0, s>>>(W, (R *)c->us, g, invn3, tw)));
         } else {
-          DNS_TIMED(c, sl + 1, (k_ypass<N, R, T_R, true><<<gx, bx, 0, s>>>(W, 6, g, tw, g.mW)));
+          DNS_TIMED(c, sl + 1, (k_ypass<N, R, TT, true><<<gx, bx, 0, s>>>(W, 6, g, tw, g.mW)));
           if (rk < 3)
-            DNS_TIMED(c, sl + 2, (k_zfused<N, R, PPW, 1, R, 1, true><<<gz, bz, 0, s>>>(W, (R *)c->us, g, invn3, tw)));
+            DNS_TIMED(c, sl + 2, (k_zfused<N, R, PP, 1, R, 1, true><<<gz, bz, 0, s>>>(W, (R *)c->us, g, invn3, tw)));
           else
-            DNS_TIMED(c, sl + 2, (k_zfused<N, R, PPW, 2, R, 1, true><<<gz, bz, 0, s>>>(W, (R *)c->us, g, invn3, tw)));
+            DNS_TIMED(c, sl + 2, (k_zfused<N, R, PP, 2, R, 1, true><<<gz, bz, 0, s>>>(W, (R *)c->us, g, invn3, tw)));
         }
         DNS_TIMED(c, sl + 3,
-                  (k_ypass<N, R, T_R, false, 1, false, true><<<gy_trunc, bx, 0, s>>>(W, 3, g, tw, g.mW)));
+                  (k_ypass<N, R, TT, false, 1, false, true><<<gy_trunc, bx, 0, s>>>(W, 3, g, tw, g.mW)));
         // E fused with the next stage's A (the next step's stage 1 after rk 3)
         switch (rk) {
           case 0:
-            DNS_TIMED(c, sl + 5, (k_xfwd_rk_xinv<N, R, T_R, 0, SR><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, nullptr, g, q, tw)));
+            DNS_TIMED(c, sl + 5, (k_xfwd_rk_xinv<N, R, TT, 0, SR><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, nullptr, g, q, tw)));
             break;
           case 1:
-            DNS_TIMED(c, sl + 5, (k_xfwd_rk_xinv<N, R, T_R, 1, SR><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, nullptr, g, q, tw)));
+            DNS_TIMED(c, sl + 5, (k_xfwd_rk_xinv<N, R, TT, 1, SR><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, nullptr, g, q, tw)));
             break;
           case 2:
-            DNS_TIMED(c, sl + 5, (k_xfwd_rk_xinv<N, R, T_R, 2, SR><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, nullptr, g, q, tw)));
+            DNS_TIMED(c, sl + 5, (k_xfwd_rk_xinv<N, R, TT, 2, SR><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, nullptr, g, q, tw)));
             break;
           default:
-            DNS_TIMED(c, sl + 5, (k_xfwd_rk_xinv<N, R, T_R, 3, SR><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, ph, g, q, tw)));
+            DNS_TIMED(c, sl + 5, (k_xfwd_rk_xinv<N, R, TT, 3, SR><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, ph, g, q, tw)));
             break;
         }
         if (rk < 3 && c->rec[rk])

@@ -99,6 +99,16 @@ int dns_set_stale_velocity(dns_ctx *ctx, const double *uvw);
 int dns_rfft3(dns_ctx *ctx, const double *real, double *spec);
 int dns_irfft3(dns_ctx *ctx, const double *spec, double *real);
 
+/* Launch shapes.  A context's kernels can run with several block shapes
+ * (strided-pass column tile width, z-pass pencils per block) that all give
+ * bitwise identical results; grids up to 256 carry 4 shapes, larger grids 1.
+ * dns_autotune times each shape for `steps` steps from the current state,
+ * keeps the fastest and restores the state. */
+int dns_launch_shapes(dns_ctx *ctx);
+int dns_set_launch_shape(dns_ctx *ctx, int shape);
+int dns_get_launch_shape(dns_ctx *ctx);
+int dns_autotune(dns_ctx *ctx, int steps, int *best_shape);
+
 /* Kernel timing.  When on, every kernel launched by dns_step is bracketed
  * by HIP events on the context's stream; times accumulate per slot
  * 6*stage + pass (stage 0..3; pass 0..5 = x-inverse with curl, y-inverse,
@@ -151,6 +161,11 @@ int dns_batch_diagnostics(dns_batch *b, int member, double *energy, double *enst
 int dns_batch_get_spectral(dns_batch *b, int member, double *uvw_hat);
 int dns_batch_get_stale_velocity(dns_batch *b, int member, double *uvw);
 int dns_batch_member_n(dns_batch *b, int member);
+/* Autotune every grid size of the batch (as dns_autotune); the step graph
+ * is recaptured with the chosen shapes.  dns_batch_launch_shape reports the
+ * shape used for a member's grid size. */
+int dns_batch_autotune(dns_batch *b, int steps);
+int dns_batch_launch_shape(dns_batch *b, int member);
 long long dns_batch_device_bytes(const dns_batch *b);
 
 /* Bytes of device memory held by the context. */

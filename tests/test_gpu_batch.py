@@ -58,3 +58,49 @@ def test_ensemble_config_shape():
                 assert np.array_equal(b.get_spectral(m), s.get_spectral())
         e = [b.diagnostics(m)[0] for m in range(len(ns))]
         assert all(np.isfinite(e))
+
+
+def test_launch_shapes_bitwise_identical():
+    n = 64
+    u = random_field(n, 31)
+    ref = None
+    with dns_amd.Solver(n, 0.01, 0.004, "mixed") as s:
+        assert s.launch_shapes() == 4
+        for shape in range(4):
+            s.set_launch_shape(shape)
+            s.load_velocity(u)
+            s.step(4)
+            got = s.get_spectral()
+            if ref is None:
+                ref = got
+            assert np.array_equal(got, ref), shape
+
+
+def test_graph_replay_equals_direct_launches():
+    n = 48
+    u = random_field(n, 32)
+    with dns_amd.Solver(n, 0.01, 0.004) as a, dns_amd.Solver(n, 0.01, 0.004) as b:
+        a.load_velocity(u)
+        b.load_velocity(u)
+        b.set_kernel_timing(True)  # timing forces direct launches
+        for _ in range(3):
+            a.step(3)
+            b.step(3)
+            assert np.array_equal(a.get_spectral(), b.get_spectral())
+        assert sum(cnt for _, cnt in b.kernel_timing().values()) > 0
+
+
+def test_autotuned_batch_still_matches_single_runs():
+    ns = [64, 96, 64, 128]
+    nus = [0.01, 0.005, 0.0025, 0.002]
+    with dns_amd.Batch(ns, nus, 2e-3, "mixed") as b:
+        for m, n in enumerate(ns):
+            b.load_velocity(m, _ic(n, m))
+        shapes = b.autotune(2)
+        assert all(0 <= v < 4 for v in shapes)
+        b.step(6)
+        for m in (0, 1, 3):
+            with dns_amd.Solver(ns[m], nus[m], 2e-3, "mixed") as s:
+                s.load_velocity(_ic(ns[m], m))
+                s.step(6)
+                assert np.array_equal(b.get_spectral(m), s.get_spectral())
