@@ -49,6 +49,7 @@ def gpu_run(dns_amd, n, nu, dt, precision, u, warmup, steps):
     from dns_amd import roofline
     s = dns_amd.Solver(n, nu, dt, precision)
     s.load_velocity(u)
+    shape = s.autotune(2)  # fastest of the bitwise-identical launch shapes
     if warmup:
         s.step(warmup)
     s.diagnostics()
@@ -70,6 +71,7 @@ def gpu_run(dns_amd, n, nu, dt, precision, u, warmup, steps):
     kern_ms = sum(d["ms"] for d in per_pass.values())
     s.close()
     return {"steps_per_s": steps / wall, "ms_per_step": 1e3 * wall / steps, "energy": e, "enstrophy": z,
+            "launch_shape": shape,
             "kernel_ms_per_step": kern_ms / steps, "ms_per_step_with_events": 1e3 * wall_ev / steps,
             "passes": per_pass}
 
@@ -272,6 +274,8 @@ def main():
         print("bench: mixed", file=sys.stderr, flush=True)
         rmx = gpu_run(dns_amd, n, nu, dt, "mixed", u, a.warmup, a.steps)
         mixed = {"steps_per_s": rmx["steps_per_s"], "ms_per_step": rmx["ms_per_step"],
+                 "launch_shape": rmx["launch_shape"],
+                 "step_bytes": roofline.step_bytes_fused(n, "mixed"),
                  "speedup_vs_fp64": rmx["steps_per_s"] / r64["steps_per_s"],
                  "energy_rel_err_vs_fp64": abs(rmx["energy"] - r64["energy"]) / r64["energy"],
                  "enstrophy_rel_err_vs_fp64": abs(rmx["enstrophy"] - r64["enstrophy"]) / r64["enstrophy"],
@@ -292,7 +296,7 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"Taylor-Green vortex n={n}, nu={nu:.6g}, dt={dt:g}, one RK4 step (FP64)",
-                   "n": n, "nu": nu, "dt": dt, "precision": "fp64"},
+                   "n": n, "nu": nu, "dt": dt, "precision": "fp64", "launch_shape": r64["launch_shape"]},
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": dom["GBs"], "peak": roofline.PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": dom["frac"], "traffic": traffic,
                      "bytes_per_launch": dom["bytes_per_launch"], "avg_launch_ms": dom["avg_ms"]},

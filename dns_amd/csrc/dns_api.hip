@@ -123,7 +123,7 @@ static int ctx_create(dns_ctx **out, int n, int members, const double *nu, const
   const size_t M = (size_t)members, C64 = sizeof(double2) * c->g.fs;
   DNS_TRY(alloc(c, &c->uhn, M * 3 * C64));
   DNS_TRY(alloc(c, &c->uhr, M * 3 * impl->sr_size() * c->g.fs));
-  DNS_TRY(alloc(c, &c->acc, M * 3 * C64));
+  DNS_TRY(alloc(c, &c->acc, M * impl->acc_bytes(c->g.fs)));
   DNS_TRY(alloc(c, &c->W, M * 6 * 2 * impl->r_size() * c->g.fs));
   DNS_TRY(alloc(c, &c->us, M * 3 * impl->r_size() * c->g.rs));
   DNS_TRY(alloc(c, (void **)&c->part, sizeof(double) * 4 * DIAG_BLOCKS));

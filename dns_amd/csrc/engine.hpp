@@ -92,6 +92,7 @@ namespace dnsk {
 struct Impl {
   virtual ~Impl() {}
   virtual size_t sr_size() const = 0;  // bytes per complex element of uhr
+  virtual size_t acc_bytes(long fs) const = 0;
   virtual size_t r_size() const = 0;   // bytes per real element of W/us
   virtual int make_twiddles(dns_ctx *c) = 0;
   virtual int load(dns_ctx *c, int m, const double *uvw) = 0;
@@ -176,15 +177,20 @@ namespace dnsk {
 
 template <int N, typename R, typename SR> struct Engine final : Impl {
   static constexpr int E = Sched<N>::E, TPP = N / E;
-  // Column-tile widths: 128-byte row segments, at most 1024 threads.
-  static constexpr int T_R = (sizeof(R) == 8 ? 8 : 16) * TPP > 1024 ? 1024 / TPP : (sizeof(R) == 8 ? 8 : 16);
-  static constexpr int T_D = 8 * TPP > 1024 ? 1024 / TPP : 8;
+  // Column-tile widths: 128-byte row segments; double blocks stay at 512
+  // threads or fewer (256 registers per thread, no spills at n=1024),
+  // float blocks at 1024.
+  static constexpr int T_R = sizeof(R) == 8 ? (8 * TPP > 512 ? 512 / TPP : 8) : (16 * TPP > 1024 ? 1024 / TPP : 16);
+  static constexpr int T_D = 8 * TPP > 512 ? 512 / TPP : 8;
   // z pencils per block: one wave (or one pencil when a pencil needs more
   // than a wave), so the pass's barriers never couple independent pencils.
   static constexpr int PPW = TPP >= 64 ? 1 : 64 / TPP;
   static constexpr int NF = N / 2 + 1;
 
   size_t sr_size() const override { return sizeof(cx<SR>); }
+  // bytes per member of the accumulator region: 3 double fields, or in
+  // mixed precision 12 float fields (Un copy and the increments dU0..dU2)
+  size_t acc_bytes(long fs) const override { return sizeof(SR) == 4 ? 12 * sizeof(float2) * fs : 3 * sizeof(double2) * fs; }
   size_t r_size() const override { return sizeof(R); }
 
   static const cx<R> *twR(dns_ctx *c) { return (const cx<R> *)(sizeof(R) == 8 ? c->tw64 : c->tw32); }
@@ -260,8 +266,8 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
 
   // Launch shapes selectable at run time (all give identical arithmetic):
   // bit 0 halves the strided passes' column tile, bit 1 doubles the z pass's
-  // pencils per block.  Only grids up to 256 carry the alternatives.
-  static constexpr bool TUNABLE = N <= 256;
+  // pencils per block.  Only grids up to 512 carry the alternatives.
+  static constexpr bool TUNABLE = N <= 512;
   static constexpr int T_H = T_R >= 4 ? T_R / 2 : T_R;
   static constexpr int PPW2 = 2 * PPW;
   int variants() const override { return TUNABLE ? 4 : 1; }

@@ -407,6 +407,11 @@ __global__ __launch_bounds__(PPW *(N / Sched<N>::E)) void k_zc2r(const cx<R> *__
 //  STAGE 0  : Ur = Un;  Ur' = Un + b dU;  acc = Un + a dU
 //  STAGE 1,2: Ur' = Un + b dU;  acc += a dU
 //  STAGE 3  : Un = acc + a dU                     (main.c:400-420)
+// Mixed precision (SR = float) keeps the accumulation in double but stores
+// its terms instead of the running sum: `acc` then holds, per member, a
+// float copy of Un and the float increments dU_0..dU_2 (each already only
+// float-accurate, being made from the float nonlinear term), and stage 3
+// forms Un + a0 dU0 + a1 dU1 + a2 dU2 + a3 dU3 in double in main.c's order.
 // The epilogue runs in double whatever R is.  phat (optional) receives P.
 // The new state (Ur' for stages 0..2, Un for stage 3) is returned in
 // U,V,Wv as the next stage's pass A would read it back from memory.
@@ -422,9 +427,11 @@ __device__ __forceinline__ void forward_x_rk(const cx<R> *__restrict__ W, cx<SR>
   constexpr int E = Sched<N>::E, TPP = N / E;
   const long off = (long)j * g.nfp + k, istr = (long)N * g.nfp;
   const TileSlot<N, T> slot{p};
+  constexpr bool MIXED_STATE = sizeof(SR) == 4;
   W += blockIdx.z * g.mW;
   uhr += blockIdx.z * g.mS;
   uhn += blockIdx.z * g.mS;
+  float2 *inc = (float2 *)acc + blockIdx.z * 4 * g.mS;  // mixed: [Un | dU0 | dU1 | dU2]
   acc += blockIdx.z * g.mS;
   if (phat) phat += blockIdx.z * g.fs;
   const double2 nd = q.nudt[blockIdx.z];
@@ -480,7 +487,34 @@ __device__ __forceinline__ void forward_x_rk(const cx<R> *__restrict__ W, cx<SR>
       else ur = cvt<double, SR>(uhr[lf]);
       d[f].x -= P.x * kv[f] + vk * ur.x;
       d[f].y -= P.y * kv[f] + vk * ur.y;
-      if constexpr (STAGE == 0) {
+      if constexpr (MIXED_STATE) {
+        const double2 du = d[f];
+        if constexpr (STAGE == 0) {
+          const cx<SR> nr = cvt<SR, double>(make_double2(ur.x + q.b * du.x, ur.y + q.b * du.y));
+          st<SR>(uhr + lf, nr);
+          st<float>(inc + lf, cvt<float, double>(ur));
+          st<float>(inc + 3 * g.fs + lf, cvt<float, double>(du));
+          nxt[f] = cvt<R, SR>(nr);
+        } else if constexpr (STAGE < 3) {
+          const double2 un = cvt<double, float>(inc[lf]);
+          const cx<SR> nr = cvt<SR, double>(make_double2(un.x + q.b * du.x, un.y + q.b * du.y));
+          st<SR>(uhr + lf, nr);
+          st<float>(inc + (3 + 3 * STAGE) * g.fs + lf, cvt<float, double>(du));
+          nxt[f] = cvt<R, SR>(nr);
+        } else {
+          const double a0 = 1 / 6.0, a1 = 1 / 3.0, a2 = 1 / 3.0;  // main.c:15
+          const double2 un = uhn[lf];
+          const double2 d0 = cvt<double, float>(inc[3 * g.fs + lf]);
+          const double2 d1 = cvt<double, float>(inc[6 * g.fs + lf]);
+          const double2 d2 = cvt<double, float>(inc[9 * g.fs + lf]);
+          double2 nn = make_double2(un.x + a0 * d0.x, un.y + a0 * d0.y);
+          nn = make_double2(nn.x + a1 * d1.x, nn.y + a1 * d1.y);
+          nn = make_double2(nn.x + a2 * d2.x, nn.y + a2 * d2.y);
+          nn = make_double2(nn.x + q.a * du.x, nn.y + q.a * du.y);
+          st<double>(uhn + lf, nn);
+          nxt[f] = cvt<R, double>(nn);
+        }
+      } else if constexpr (STAGE == 0) {
         const cx<SR> nr = cvt<SR, double>(make_double2(ur.x + q.b * d[f].x, ur.y + q.b * d[f].y));
         st<SR>(uhr + lf, nr);
         st<double>(acc + lf, make_double2(ur.x + q.a * d[f].x, ur.y + q.a * d[f].y));

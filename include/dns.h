@@ -101,7 +101,7 @@ int dns_irfft3(dns_ctx *ctx, const double *spec, double *real);
 
 /* Launch shapes.  A context's kernels can run with several block shapes
  * (strided-pass column tile width, z-pass pencils per block) that all give
- * bitwise identical results; grids up to 256 carry 4 shapes, larger grids 1.
+ * bitwise identical results; grids up to 512 carry 4 shapes, larger grids 1.
  * dns_autotune times each shape for `steps` steps from the current state,
  * keeps the fastest and restores the state. */
 int dns_launch_shapes(dns_ctx *ctx);

@@ -65,7 +65,7 @@ def test_launch_shapes_bitwise_identical():
     u = random_field(n, 31)
     ref = None
     with dns_amd.Solver(n, 0.01, 0.004, "mixed") as s:
-        assert s.launch_shapes() == 4
+        assert s.launch_shapes() == 4  # grids up to 512
         for shape in range(4):
             s.set_launch_shape(shape)
             s.load_velocity(u)
