@@ -177,11 +177,21 @@ namespace dnsk {
 
 template <int N, typename R, typename SR> struct Engine final : Impl {
   static constexpr int E = Sched<N>::E, TPP = N / E;
-  // Column-tile widths: 128-byte row segments; double blocks stay at 512
-  // threads or fewer (256 registers per thread, no spills at n=1024),
-  // float blocks at 1024.
-  static constexpr int T_R = sizeof(R) == 8 ? (8 * TPP > 512 ? 512 / TPP : 8) : (16 * TPP > 1024 ? 1024 / TPP : 16);
+  // Column tiles of the strided passes; every block has at most 512
+  // threads (256 registers per thread, no spills at n=1024).
+  //  y passes and double x passes: 8 columns (128-byte row segments in
+  //  double, 64 in float);
+  //  float x passes (stages 1-3): 16 columns (128 bytes) with 16 elements
+  //  per thread where the schedule allows (powers of two from 256 on).
   static constexpr int T_D = 8 * TPP > 512 ? 512 / TPP : 8;
+  static constexpr int T_Y = T_D;
+  // forward (truncated) y pass in float: 16 columns, up to 1024 threads
+  // (its register needs are small enough)
+  static constexpr int T_YF = sizeof(R) == 8 ? T_D : (16 * TPP > 1024 ? 1024 / TPP : 16);
+  static constexpr bool PIPE_YF = sizeof(R) == 8;
+  static constexpr int E_X = (sizeof(R) == 4 && E == 8 && N >= 256) ? 16 : E;
+  static constexpr int TPP_X = N / E_X;
+  static constexpr int T_X = sizeof(R) == 8 ? T_D : (16 * TPP_X > 512 ? 512 / TPP_X : 16);
   // z pencils per block: one wave (or one pencil when a pencil needs more
   // than a wave), so the pass's barriers never couple independent pencils.
   static constexpr int PPW = TPP >= 64 ? 1 : 64 / TPP;
@@ -251,8 +261,8 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
 
   int prepare(dns_ctx *c) override {
     if (c->w_ready) return DNS_OK;
-    const dim3 gx = grid_x(T_R, c->members), bx(T_R * TPP);
-    DNS_TIMED(c, 0, (k_xinv_curl<N, R, T_R, false, double><<<gx, bx, 0, c->stream>>>(
+    const dim3 gx = grid_x(T_X, c->members), bx(T_X * TPP_X);
+    DNS_TIMED(c, 0, (k_xinv_curl<N, R, T_X, false, double, 1, E_X><<<gx, bx, 0, c->stream>>>(
                         (const double2 *)c->uhn, (cx<R> *)c->W, c->g, twR(c))));
     c->w_ready = 1;
     return DNS_OK;
@@ -268,7 +278,9 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
   // bit 0 halves the strided passes' column tile, bit 1 doubles the z pass's
   // pencils per block.  Only grids up to 512 carry the alternatives.
   static constexpr bool TUNABLE = N <= 512;
-  static constexpr int T_H = T_R >= 4 ? T_R / 2 : T_R;
+  static constexpr int T_YH = T_Y >= 4 ? T_Y / 2 : T_Y;
+  static constexpr int T_YFH = T_YF >= 4 ? T_YF / 2 : T_YF;
+  static constexpr int T_XH = T_X >= 4 ? T_X / 2 : T_X;
   static constexpr int PPW2 = 2 * PPW;
   int variants() const override { return TUNABLE ? 4 : 1; }
 
@@ -282,13 +294,13 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
   int step_launch(dns_ctx *c, long nsteps) {
     if constexpr (TUNABLE) {
       switch (c->variant) {
-        case 1: return step_body<T_H, PPW>(c, nsteps);
-        case 2: return step_body<T_R, PPW2>(c, nsteps);
-        case 3: return step_body<T_H, PPW2>(c, nsteps);
+        case 1: return step_body<T_YH, T_YFH, T_XH, PPW>(c, nsteps);
+        case 2: return step_body<T_Y, T_YF, T_X, PPW2>(c, nsteps);
+        case 3: return step_body<T_YH, T_YFH, T_XH, PPW2>(c, nsteps);
         default: break;
       }
     }
-    return step_body<T_R, PPW>(c, nsteps);
+    return step_body<T_Y, T_YF, T_X, PPW>(c, nsteps);
   }
 
   // One step captured once into a graph (per launch shape) and replayed.
@@ -317,7 +329,11 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
     return DNS_OK;
   }
 
-  template <int TT, int PP> int step_body(dns_ctx *c, long nsteps) {
+  // TY: inverse y-pass tile; TYF: forward y-pass tile; TX: x-pass tile
+  // (E_X elements per thread for stages 1-3; stage 4 uses the schedule's E
+  // and the y-pass tile, which fits the register budget of its five state
+  // loads per element); PP: z pencils per block.
+  template <int TY, int TYF, int TX, int PP> int step_body(dns_ctx *c, long nsteps) {
     hipStream_t s = c->stream;
     const Geo g = c->g;
     const int M = c->members;
@@ -328,9 +344,10 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
     const R invn3 = (R)c->invn3;
     static const double a_rk[4] = {1 / 6.0, 1 / 3.0, 1 / 3.0, 1 / 6.0};  // main.c:15
     static const double b_rk[4] = {0.5, 0.5, 1.0, 0.0};                 // main.c:16
-    const dim3 gx = grid_x(TT, M), bx(TT * TPP);
+    const dim3 gy = grid_x(TY, M), by(TY * TPP);
+    const dim3 gx = grid_x(TX, M), bx(TX * TPP_X);
     const dim3 gz(N * N / PP, 1, M), bz(PP * TPP);
-    const dim3 gy_trunc((g.nk + TT - 1) / TT, N, M);  // forward y pass: dealiased kz columns only
+    const dim3 gy_trunc((g.nk + TYF - 1) / TYF, N, M), byf(TYF * TPP);  // dealiased kz columns only
     for (long st = 0; st < nsteps; st++) {
       double2 *ph = (c->capture && st == nsteps - 1) ? (double2 *)c->phat : nullptr;
       for (int rk = 0; rk < 4; rk++) {
@@ -338,30 +355,30 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
         const int sl = DNS_NPASS * rk;
         // pass A of this stage ran fused into the previous stage's E
         if (rk == 0) {
-          DNS_TIMED(c, sl + 1, (k_ypass<N, R, TT, true><<<gx, bx, 0, s>>>(W + 3 * g.fs, 3, g, tw, g.mW)));
+          DNS_TIMED(c, sl + 1, (k_ypass<N, R, TY, true, 1, true><<<gy, by, 0, s>>>(W + 3 * g.fs, 3, g, tw, g.mW)));
           DNS_TIMED(c, sl + 2, (k_zfused<N, R, PP, 0, R, 1, true><<<gz, bz, 0, s>>>(W, (R *)c->us, g, invn3, tw)));
         } else {
-          DNS_TIMED(c, sl + 1, (k_ypass<N, R, TT, true><<<gx, bx, 0, s>>>(W, 6, g, tw, g.mW)));
+          DNS_TIMED(c, sl + 1, (k_ypass<N, R, TY, true, 1, true><<<gy, by, 0, s>>>(W, 6, g, tw, g.mW)));
           if (rk < 3)
             DNS_TIMED(c, sl + 2, (k_zfused<N, R, PP, 1, R, 1, true><<<gz, bz, 0, s>>>(W, (R *)c->us, g, invn3, tw)));
           else
             DNS_TIMED(c, sl + 2, (k_zfused<N, R, PP, 2, R, 1, true><<<gz, bz, 0, s>>>(W, (R *)c->us, g, invn3, tw)));
         }
         DNS_TIMED(c, sl + 3,
-                  (k_ypass<N, R, TT, false, 1, false, true><<<gy_trunc, bx, 0, s>>>(W, 3, g, tw, g.mW)));
+                  (k_ypass<N, R, TYF, false, 1, PIPE_YF, true><<<gy_trunc, byf, 0, s>>>(W, 3, g, tw, g.mW)));
         // E fused with the next stage's A (the next step's stage 1 after rk 3)
         switch (rk) {
           case 0:
-            DNS_TIMED(c, sl + 5, (k_xfwd_rk_xinv<N, R, TT, 0, SR><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, nullptr, g, q, tw)));
+            DNS_TIMED(c, sl + 5, (k_xfwd_rk_xinv<N, R, TX, 0, SR, 1, 0, E_X><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, nullptr, g, q, tw)));
             break;
           case 1:
-            DNS_TIMED(c, sl + 5, (k_xfwd_rk_xinv<N, R, TT, 1, SR><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, nullptr, g, q, tw)));
+            DNS_TIMED(c, sl + 5, (k_xfwd_rk_xinv<N, R, TX, 1, SR, 1, 0, E_X><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, nullptr, g, q, tw)));
             break;
           case 2:
-            DNS_TIMED(c, sl + 5, (k_xfwd_rk_xinv<N, R, TT, 2, SR><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, nullptr, g, q, tw)));
+            DNS_TIMED(c, sl + 5, (k_xfwd_rk_xinv<N, R, TX, 2, SR, 1, 0, E_X><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, nullptr, g, q, tw)));
             break;
           default:
-            DNS_TIMED(c, sl + 5, (k_xfwd_rk_xinv<N, R, TT, 3, SR><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, ph, g, q, tw)));
+            DNS_TIMED(c, sl + 5, (k_xfwd_rk_xinv<N, R, TY, 3, SR><<<gy, by, 0, s>>>(W, uhr, uhn, acc, ph, g, q, tw)));
             break;
         }
         if (rk < 3 && c->rec[rk])
@@ -416,9 +433,9 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
     DNS_HIP(hipMemcpyAsync(phys, in, sizeof(double) * g.rs, hipMemcpyHostToDevice, s));
     k_zr2c<N, R, PPW><<<N * N / PPW, PPW * TPP, 0, s>>>(phys, spec, 1, g, twR(c));
     DNS_LAUNCHED();
-    k_ypass<N, R, T_R, false><<<grid_x(T_R), T_R * TPP, 0, s>>>(spec, 1, g, twR(c));
+    k_ypass<N, R, T_Y, false><<<grid_x(T_Y), T_Y * TPP, 0, s>>>(spec, 1, g, twR(c));
     DNS_LAUNCHED();
-    k_xplain<N, R, T_R, false, R><<<grid_x(T_R), T_R * TPP, 0, s>>>(spec, spec, 1, g, twR(c));
+    k_xplain<N, R, T_Y, false, R><<<grid_x(T_Y), T_Y * TPP, 0, s>>>(spec, spec, 1, g, twR(c));
     DNS_LAUNCHED();
     double2 *specd = (double2 *)spec;
     if (sizeof(R) != 8) {  // widen into the region after the float spectrum
@@ -443,9 +460,9 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
     DNS_HIP(hipMemset2DAsync(specd, sizeof(double2) * g.nfp, 0, sizeof(double2) * g.nfp, (size_t)N * N, s));
     DNS_HIP(hipMemcpy2DAsync(specd, sizeof(double2) * g.nfp, in, sizeof(double2) * NF, sizeof(double2) * NF,
                              (size_t)N * N, hipMemcpyHostToDevice, s));
-    k_xplain<N, R, T_R, true, double><<<grid_x(T_R), T_R * TPP, 0, s>>>(specd, tmpc, 1, g, twR(c));
+    k_xplain<N, R, T_Y, true, double><<<grid_x(T_Y), T_Y * TPP, 0, s>>>(specd, tmpc, 1, g, twR(c));
     DNS_LAUNCHED();
-    k_ypass<N, R, T_R, true><<<grid_x(T_R), T_R * TPP, 0, s>>>(tmpc, 1, g, twR(c));
+    k_ypass<N, R, T_Y, true><<<grid_x(T_Y), T_Y * TPP, 0, s>>>(tmpc, 1, g, twR(c));
     DNS_LAUNCHED();
     k_zc2r<N, R, PPW><<<N * N / PPW, PPW * TPP, 0, s>>>(tmpc, phys, g, (R)1, twR(c));
     DNS_LAUNCHED();

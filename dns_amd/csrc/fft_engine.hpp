@@ -159,6 +159,24 @@ template <typename R, int RAD, bool INV> __device__ __forceinline__ void dft(cx<
   else dft8<R, INV>(a);
 }
 
+// Twiddle table staged in LDS by the whole block.  LDS reads are waited for
+// separately from global memory operations, so a transform that follows a
+// batch of global stores does not stall until those stores complete (a
+// global twiddle load would: loads and stores share one counter).
+#ifndef DNS_LDS_TWIDDLES
+#define DNS_LDS_TWIDDLES 1
+#endif
+template <int N, typename R>
+__device__ __forceinline__ const cx<R> *stage_twiddles(cx<R> *twl, const cx<R> *__restrict__ tw) {
+  if constexpr (DNS_LDS_TWIDDLES) {
+    for (int i = threadIdx.x; i < tw_size<N>(); i += blockDim.x) twl[i] = tw[i];
+    __syncthreads();
+    return twl;
+  } else {
+    return tw;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // LDS slot maps.  One dummy row every 8 positions breaks the power-of-two
 // strides of the first stages' scattered writes.
@@ -179,10 +197,13 @@ template <int N> struct PencilSlot {
 // In-register FFT of one pencil.  v[c] holds position t + TPP*c on entry
 // and the transform's coefficient at the same position on exit.
 // All threads of the block must call it together (it uses __syncthreads).
-template <int N, typename R, bool INV, class Slot>
-__device__ __forceinline__ void fft_pencil(cx<R> (&v)[Sched<N>::E], int t, cx<R> *lds, Slot slot,
+// E (elements per thread) defaults to the schedule's; any multiple of every
+// radix of the schedule works (fewer threads per pencil, more registers).
+template <int N, typename R, bool INV, class Slot, int E>
+__device__ __forceinline__ void fft_pencil(cx<R> (&v)[E], int t, cx<R> *lds, Slot slot,
                                            const cx<R> *__restrict__ tw) {
-  constexpr int E = Sched<N>::E, NS = Sched<N>::NS, TPP = N / E;
+  constexpr int NS = Sched<N>::NS, TPP = N / E;
+  static_for<0, NS>([&](auto S) { static_assert(E % Sched<N>::r[decltype(S)::value] == 0, "radix must divide E"); });
   static_for<0, NS>([&](auto S) {
     constexpr int s = decltype(S)::value;
     constexpr int r = Sched<N>::r[s];

@@ -21,6 +21,7 @@
 //                    k_xfwd_rk_xinv fuses it with the next stage's A
 #pragma once
 #include "fft_engine.hpp"
+#include <type_traits>
 
 namespace dnsk {
 
@@ -63,12 +64,14 @@ template <int N> __device__ __forceinline__ int wave(int i) { return i < N / 2 ?
 // TRUNC (forward transform of the nonlinear term only): the columns
 // k >= g.nk are not read and only rows inside the dealiased box are written;
 // the grid then covers ceil(nk/T) column tiles.
-template <int N, typename R, int T, bool INV, int MINW = 1, bool PIPE = false, bool TRUNC = false>
-__global__ __launch_bounds__(T *(N / Sched<N>::E), MINW) void k_ypass(cx<R> *__restrict__ data, int nfields,
-                                                                       Geo g, const cx<R> *__restrict__ tw,
-                                                                       long mstride = 0) {
-  constexpr int E = Sched<N>::E, TPP = N / E;
+template <int N, typename R, int T, bool INV, int MINW = 1, bool PIPE = false, bool TRUNC = false,
+          int E = Sched<N>::E>
+__global__ __launch_bounds__(T *(N / E), MINW) void k_ypass(cx<R> *__restrict__ data, int nfields, Geo g,
+                                                           const cx<R> *__restrict__ tw, long mstride = 0) {
+  constexpr int TPP = N / E;
   __shared__ cx<R> lds[T * padlen<N>()];
+  __shared__ cx<R> twl[tw_size<N>()];
+  tw = stage_twiddles<N, R>(twl, tw);
   const int p = threadIdx.x % T, t = threadIdx.x / T;
   const int k = blockIdx.x * T + p;
   const bool ok = k < (TRUNC ? g.nk : N / 2 + 1);
@@ -108,6 +111,8 @@ __global__ __launch_bounds__(T *(N / Sched<N>::E)) void k_xplain(const cx<RI> *i
                                                                   const cx<R> *__restrict__ tw) {
   constexpr int E = Sched<N>::E, TPP = N / E;
   __shared__ cx<R> lds[T * padlen<N>()];
+  __shared__ cx<R> twl[tw_size<N>()];
+  tw = stage_twiddles<N, R>(twl, tw);
   const int p = threadIdx.x % T, t = threadIdx.x / T;
   const int k = blockIdx.x * T + p;
   const bool ok = k < N / 2 + 1;
@@ -133,12 +138,11 @@ __global__ __launch_bounds__(T *(N / Sched<N>::E)) void k_xplain(const cx<RI> *i
 // with I*z = (-Im z, Re z).  Blocks own x-pencil tiles: grid (ceil(nf/T), n),
 // blockIdx.y = j.  U, V, Wv hold the spectral velocity at the thread's
 // positions i = t + TPP*c of pencil (j, k).
-template <int N, typename R, int T, bool WITH_U>
-__device__ __forceinline__ void curl_inverse_x(cx<R> (&U)[Sched<N>::E], cx<R> (&V)[Sched<N>::E],
-                                               cx<R> (&Wv)[Sched<N>::E], cx<R> *__restrict__ W, Geo g, int t,
-                                               int p, int j, int k, bool ok, cx<R> *lds,
-                                               const cx<R> *__restrict__ tw) {
-  constexpr int E = Sched<N>::E, TPP = N / E;
+template <int N, typename R, int T, bool WITH_U, int E>
+__device__ __forceinline__ void curl_inverse_x(cx<R> (&U)[E], cx<R> (&V)[E], cx<R> (&Wv)[E],
+                                               cx<R> *__restrict__ W, Geo g, int t, int p, int j, int k, bool ok,
+                                               cx<R> *lds, const cx<R> *__restrict__ tw) {
+  constexpr int TPP = N / E;
   const long off = (long)j * g.nfp + k, istr = (long)N * g.nfp;
   const R ky = (R)wave<N>(j), kz = (R)k;
   const TileSlot<N, T> slot{p};
@@ -190,12 +194,14 @@ __device__ __forceinline__ void curl_inverse_x(cx<R> (&U)[Sched<N>::E], cx<R> (&
   }
 }
 
-template <int N, typename R, int T, bool WITH_U, typename SI, int MINW = 1>
-__global__ __launch_bounds__(T *(N / Sched<N>::E), MINW) void k_xinv_curl(const cx<SI> *__restrict__ uh,
-                                                                     cx<R> *__restrict__ W, Geo g,
-                                                                     const cx<R> *__restrict__ tw) {
-  constexpr int E = Sched<N>::E, TPP = N / E;
+template <int N, typename R, int T, bool WITH_U, typename SI, int MINW = 1, int E = Sched<N>::E>
+__global__ __launch_bounds__(T *(N / E), MINW) void k_xinv_curl(const cx<SI> *__restrict__ uh,
+                                                           cx<R> *__restrict__ W, Geo g,
+                                                           const cx<R> *__restrict__ tw) {
+  constexpr int TPP = N / E;
   __shared__ cx<R> lds[T * padlen<N>()];
+  __shared__ cx<R> twl[tw_size<N>()];
+  tw = stage_twiddles<N, R>(twl, tw);
   const int p = threadIdx.x % T, t = threadIdx.x / T;
   const int j = blockIdx.y, k = blockIdx.x * T + p;
   const bool ok = k < N / 2 + 1;
@@ -210,7 +216,7 @@ __global__ __launch_bounds__(T *(N / Sched<N>::E), MINW) void k_xinv_curl(const 
     V[c] = ok ? cvt<R, SI>(uh[g.fs + l]) : zero<R>();
     Wv[c] = ok ? cvt<R, SI>(uh[2 * g.fs + l]) : zero<R>();
   }
-  curl_inverse_x<N, R, T, WITH_U>(U, V, Wv, W, g, t, p, j, k, ok, lds, tw);
+  curl_inverse_x<N, R, T, WITH_U, E>(U, V, Wv, W, g, t, p, j, k, ok, lds, tw);
 }
 
 // ---------------------------------------------------------------------------
@@ -416,18 +422,26 @@ __global__ __launch_bounds__(PPW *(N / Sched<N>::E)) void k_zc2r(const cx<R> *__
 // The new state (Ur' for stages 0..2, Un for stage 3) is returned in
 // U,V,Wv as the next stage's pass A would read it back from memory.
 // Grid (ceil(nf/T), n): blockIdx.y = j.
-template <int N, typename R, int T, int STAGE, typename SR>
+// EPI selects the epilogue's batching: 0 = default (below), otherwise
+// BC = EPI & 63 positions per batch and prefetch of the next batch if
+// EPI & 64.  E = elements per thread (a multiple of every radix of N).
+template <int N, typename R, int T, int STAGE, typename SR, int EPI = 0, int E = Sched<N>::E>
 __device__ __forceinline__ void forward_x_rk(const cx<R> *__restrict__ W, cx<SR> *__restrict__ uhr,
                                              double2 *__restrict__ uhn, double2 *__restrict__ acc,
                                              double2 *__restrict__ phat, Geo g, RKParams q,
                                              const cx<R> *__restrict__ tw, int t, int p, int j, int k, bool ok,
-                                             cx<R> *lds, cx<R> (&U)[Sched<N>::E], cx<R> (&V)[Sched<N>::E],
-                                             cx<R> (&Wv)[Sched<N>::E]) {
+                                             cx<R> *lds, cx<R> (&U)[E], cx<R> (&V)[E], cx<R> (&Wv)[E]) {
 #pragma clang fp contract(off)
-  constexpr int E = Sched<N>::E, TPP = N / E;
+  constexpr int TPP = N / E;
   const long off = (long)j * g.nfp + k, istr = (long)N * g.nfp;
   const TileSlot<N, T> slot{p};
   constexpr bool MIXED_STATE = sizeof(SR) == 4;
+  // two positions per batch (six or nine loads; one position for the
+  // fifteen loads of the mixed stage 3 and for 16-element threads), next
+  // batch prefetched in float tiles of up to 8 elements (register budget)
+  constexpr int BC = EPI ? (EPI & 63) : ((MIXED_STATE && STAGE == 3) || E >= 16) ? 1 : 2;
+  constexpr bool PF = EPI ? (EPI & 64) != 0 : sizeof(R) == 4 && E < 16;
+  static_assert(E % BC == 0, "batch must divide the elements per thread");
   W += blockIdx.z * g.mW;
   uhr += blockIdx.z * g.mS;
   uhn += blockIdx.z * g.mS;
@@ -439,7 +453,7 @@ __device__ __forceinline__ void forward_x_rk(const cx<R> *__restrict__ W, cx<SR>
   // The y and z passes produced the nonlinear term only inside the
   // dealiased box (|ky|, kz < kmax); tiles outside it have N = 0 and skip the
   // transform (block-uniform branch).
-  const bool nl = in_box(j, N, g.nk) && (int)(blockIdx.x * T) < g.nk;
+  const bool nl = in_box(j, N, g.nk) && k - p < g.nk;
   const bool okn = k < g.nk;
   cx<R> F[3][E];
   if (nl) {
@@ -459,100 +473,159 @@ __device__ __forceinline__ void forward_x_rk(const cx<R> *__restrict__ W, cx<SR>
   const double ky = (double)wave<N>(j), kz = (double)k;
   const double nudt = nu * dt;
   const bool myz = fabs(ky) < q.kmax && fabs(kz) < q.kmax;
+  // The state values of a batch of positions (all three fields) are loaded
+  // together and the next batch's loads are issued before this batch's
+  // stores: the stores may alias the loads as far as the compiler can tell,
+  // so a per-element order would cost one memory round trip per element
+  // (and each wait would also drain the earlier stores).
+  using URT = std::conditional_t<STAGE == 0, double2, cx<SR>>;  // Ur as stored
+  using URS = std::conditional_t<STAGE == 0, double, SR>;
+  struct Batch {
+    URT ur[3][BC];      // stage state (stage 0: Un)
+    double2 un[3][BC];  // FP64 stages 1,2 and mixed stage 3: Un
+    float2 unf[3][BC];  // mixed stages 1,2: float copy of Un
+    double2 ac[3][BC];  // FP64 stages 1..3: accumulator
+    float2 dd[3][3][BC];  // mixed stage 3: dU0..dU2
+  };
+  constexpr int NB = E / BC;
+  auto load = [&](Batch &B, auto BB) {
+    constexpr int c0 = decltype(BB)::value * BC;
+    if (!ok) return;
 #pragma unroll
-  for (int c = 0; c < E; c++) {
-    U[c] = V[c] = Wv[c] = zero<R>();
-    if (!ok) continue;
-    const long l = off + (t + TPP * c) * istr;
-    const double kx = (double)wave<N>(t + TPP * c);
-    const double kk = kx * kx + ky * ky + kz * kz;
-    const double fac = (myz && fabs(kx) < q.kmax) ? dt : 0.0;
-    double2 d[3];
+    for (int i = 0; i < BC; i++) {
 #pragma unroll
-    for (int f = 0; f < 3; f++) d[f] = make_double2((double)F[f][c].x * fac, (double)F[f][c].y * fac);
-    double2 P = make_double2(0.0, 0.0);
-    if (kk > 0) {
-      P.x = (d[0].x * kx + d[1].x * ky + d[2].x * kz) / kk;
-      P.y = (d[0].y * kx + d[1].y * ky + d[2].y * kz) / kk;
+      for (int f = 0; f < 3; f++) {
+        const long lf = f * g.fs + off + (t + TPP * (c0 + i)) * istr;
+        if constexpr (STAGE == 0) B.ur[f][i] = uhn[lf];
+        else B.ur[f][i] = uhr[lf];
+        if constexpr (MIXED_STATE) {
+          if constexpr (STAGE == 1 || STAGE == 2) B.unf[f][i] = inc[lf];
+          if constexpr (STAGE == 3) {
+            B.un[f][i] = uhn[lf];
+#pragma unroll
+            for (int s = 0; s < 3; s++) B.dd[s][f][i] = inc[(3 + 3 * s) * g.fs + lf];
+          }
+        } else {
+          if constexpr (STAGE == 1 || STAGE == 2) B.un[f][i] = uhn[lf];
+          if constexpr (STAGE > 0) B.ac[f][i] = acc[lf];
+        }
+      }
     }
-    if (phat) st<double>(phat + l, P);
-    const double vk = nudt * kk;
-    const double kv[3] = {kx, ky, kz};
-    cx<R> nxt[3];
+  };
+  auto apply = [&](const Batch &B, auto BB) {
+    constexpr int c0 = decltype(BB)::value * BC;
 #pragma unroll
-    for (int f = 0; f < 3; f++) {
-      const long lf = f * g.fs + l;
-      double2 ur;
-      if constexpr (STAGE == 0) ur = uhn[lf];
-      else ur = cvt<double, SR>(uhr[lf]);
-      d[f].x -= P.x * kv[f] + vk * ur.x;
-      d[f].y -= P.y * kv[f] + vk * ur.y;
-      if constexpr (MIXED_STATE) {
+    for (int i = 0; i < BC; i++) {
+      const int c = c0 + i;
+      U[c] = V[c] = Wv[c] = zero<R>();
+      if (!ok) continue;
+      const long l = off + (t + TPP * c) * istr;
+      const double kx = (double)wave<N>(t + TPP * c);
+      const double kk = kx * kx + ky * ky + kz * kz;
+      const double fac = (myz && fabs(kx) < q.kmax) ? dt : 0.0;
+      double2 d[3];
+#pragma unroll
+      for (int f = 0; f < 3; f++) d[f] = make_double2((double)F[f][c].x * fac, (double)F[f][c].y * fac);
+      double2 P = make_double2(0.0, 0.0);
+      if (kk > 0) {
+        P.x = (d[0].x * kx + d[1].x * ky + d[2].x * kz) / kk;
+        P.y = (d[0].y * kx + d[1].y * ky + d[2].y * kz) / kk;
+      }
+      if (phat) st<double>(phat + l, P);
+      const double vk = nudt * kk;
+      const double kv[3] = {kx, ky, kz};
+      cx<R> nxt[3];
+#pragma unroll
+      for (int f = 0; f < 3; f++) {
+        const long lf = f * g.fs + l;
+        const double2 ur = cvt<double, URS>(B.ur[f][i]);
+        d[f].x -= P.x * kv[f] + vk * ur.x;
+        d[f].y -= P.y * kv[f] + vk * ur.y;
         const double2 du = d[f];
-        if constexpr (STAGE == 0) {
+        if constexpr (MIXED_STATE) {
+          if constexpr (STAGE == 0) {
+            const cx<SR> nr = cvt<SR, double>(make_double2(ur.x + q.b * du.x, ur.y + q.b * du.y));
+            st<SR>(uhr + lf, nr);
+            st<float>(inc + lf, cvt<float, double>(ur));
+            st<float>(inc + 3 * g.fs + lf, cvt<float, double>(du));
+            nxt[f] = cvt<R, SR>(nr);
+          } else if constexpr (STAGE < 3) {
+            const double2 un = cvt<double, float>(B.unf[f][i]);
+            const cx<SR> nr = cvt<SR, double>(make_double2(un.x + q.b * du.x, un.y + q.b * du.y));
+            st<SR>(uhr + lf, nr);
+            st<float>(inc + (3 + 3 * STAGE) * g.fs + lf, cvt<float, double>(du));
+            nxt[f] = cvt<R, SR>(nr);
+          } else {
+            const double a0 = 1 / 6.0, a1 = 1 / 3.0, a2 = 1 / 3.0;  // main.c:15
+            const double2 un = B.un[f][i];
+            const double2 d0 = cvt<double, float>(B.dd[0][f][i]);
+            const double2 d1 = cvt<double, float>(B.dd[1][f][i]);
+            const double2 d2 = cvt<double, float>(B.dd[2][f][i]);
+            double2 nn = make_double2(un.x + a0 * d0.x, un.y + a0 * d0.y);
+            nn = make_double2(nn.x + a1 * d1.x, nn.y + a1 * d1.y);
+            nn = make_double2(nn.x + a2 * d2.x, nn.y + a2 * d2.y);
+            nn = make_double2(nn.x + q.a * du.x, nn.y + q.a * du.y);
+            st<double>(uhn + lf, nn);
+            nxt[f] = cvt<R, double>(nn);
+          }
+        } else if constexpr (STAGE == 0) {
           const cx<SR> nr = cvt<SR, double>(make_double2(ur.x + q.b * du.x, ur.y + q.b * du.y));
           st<SR>(uhr + lf, nr);
-          st<float>(inc + lf, cvt<float, double>(ur));
-          st<float>(inc + 3 * g.fs + lf, cvt<float, double>(du));
+          st<double>(acc + lf, make_double2(ur.x + q.a * du.x, ur.y + q.a * du.y));
           nxt[f] = cvt<R, SR>(nr);
         } else if constexpr (STAGE < 3) {
-          const double2 un = cvt<double, float>(inc[lf]);
+          const double2 un = B.un[f][i];
           const cx<SR> nr = cvt<SR, double>(make_double2(un.x + q.b * du.x, un.y + q.b * du.y));
           st<SR>(uhr + lf, nr);
-          st<float>(inc + (3 + 3 * STAGE) * g.fs + lf, cvt<float, double>(du));
+          const double2 ac = B.ac[f][i];
+          st<double>(acc + lf, make_double2(ac.x + q.a * du.x, ac.y + q.a * du.y));
           nxt[f] = cvt<R, SR>(nr);
         } else {
-          const double a0 = 1 / 6.0, a1 = 1 / 3.0, a2 = 1 / 3.0;  // main.c:15
-          const double2 un = uhn[lf];
-          const double2 d0 = cvt<double, float>(inc[3 * g.fs + lf]);
-          const double2 d1 = cvt<double, float>(inc[6 * g.fs + lf]);
-          const double2 d2 = cvt<double, float>(inc[9 * g.fs + lf]);
-          double2 nn = make_double2(un.x + a0 * d0.x, un.y + a0 * d0.y);
-          nn = make_double2(nn.x + a1 * d1.x, nn.y + a1 * d1.y);
-          nn = make_double2(nn.x + a2 * d2.x, nn.y + a2 * d2.y);
-          nn = make_double2(nn.x + q.a * du.x, nn.y + q.a * du.y);
+          const double2 ac = B.ac[f][i];
+          const double2 nn = make_double2(ac.x + q.a * du.x, ac.y + q.a * du.y);
           st<double>(uhn + lf, nn);
           nxt[f] = cvt<R, double>(nn);
         }
-      } else if constexpr (STAGE == 0) {
-        const cx<SR> nr = cvt<SR, double>(make_double2(ur.x + q.b * d[f].x, ur.y + q.b * d[f].y));
-        st<SR>(uhr + lf, nr);
-        st<double>(acc + lf, make_double2(ur.x + q.a * d[f].x, ur.y + q.a * d[f].y));
-        nxt[f] = cvt<R, SR>(nr);
-      } else if constexpr (STAGE < 3) {
-        const double2 un = uhn[lf];
-        const cx<SR> nr = cvt<SR, double>(make_double2(un.x + q.b * d[f].x, un.y + q.b * d[f].y));
-        st<SR>(uhr + lf, nr);
-        const double2 ac = acc[lf];
-        st<double>(acc + lf, make_double2(ac.x + q.a * d[f].x, ac.y + q.a * d[f].y));
-        nxt[f] = cvt<R, SR>(nr);
-      } else {
-        const double2 ac = acc[lf];
-        const double2 nn = make_double2(ac.x + q.a * d[f].x, ac.y + q.a * d[f].y);
-        st<double>(uhn + lf, nn);
-        nxt[f] = cvt<R, double>(nn);
       }
+      U[c] = nxt[0];
+      V[c] = nxt[1];
+      Wv[c] = nxt[2];
     }
-    U[c] = nxt[0];
-    V[c] = nxt[1];
-    Wv[c] = nxt[2];
+  };
+  if constexpr (PF) {
+    Batch cur;
+    load(cur, std::integral_constant<int, 0>{});
+    static_for<0, NB>([&](auto BB) {
+      constexpr int b = decltype(BB)::value;
+      Batch nxb;
+      if constexpr (b + 1 < NB) load(nxb, std::integral_constant<int, b + 1>{});
+      apply(cur, BB);
+      if constexpr (b + 1 < NB) cur = nxb;
+    });
+  } else {
+    static_for<0, NB>([&](auto BB) {
+      Batch cur;
+      load(cur, BB);
+      apply(cur, BB);
+    });
   }
 }
 
-template <int N, typename R, int T, int STAGE, typename SR, int MINW = 1>
-__global__ __launch_bounds__(T *(N / Sched<N>::E), MINW) void k_xfwd_rk(const cx<R> *__restrict__ W,
+template <int N, typename R, int T, int STAGE, typename SR, int MINW = 1, int EPI = 0, int E = Sched<N>::E>
+__global__ __launch_bounds__(T *(N / E), MINW) void k_xfwd_rk(const cx<R> *__restrict__ W,
                                                                    cx<SR> *__restrict__ uhr,
                                                                    double2 *__restrict__ uhn,
                                                                    double2 *__restrict__ acc,
                                                                    double2 *__restrict__ phat, Geo g, RKParams q,
                                                                    const cx<R> *__restrict__ tw) {
-  constexpr int E = Sched<N>::E;
   __shared__ cx<R> lds[T * padlen<N>()];
+  __shared__ cx<R> twl[tw_size<N>()];
+  tw = stage_twiddles<N, R>(twl, tw);
   const int p = threadIdx.x % T, t = threadIdx.x / T;
   const int j = blockIdx.y, k = blockIdx.x * T + p;
   const bool ok = k < N / 2 + 1;
   cx<R> U[E], V[E], Wv[E];
-  forward_x_rk<N, R, T, STAGE, SR>(W, uhr, uhn, acc, phat, g, q, tw, t, p, j, k, ok, lds, U, V, Wv);
+  forward_x_rk<N, R, T, STAGE, SR, EPI, E>(W, uhr, uhn, acc, phat, g, q, tw, t, p, j, k, ok, lds, U, V, Wv);
 }
 
 // E of stage STAGE fused with A of the following stage (stage STAGE+1, or
@@ -560,21 +633,22 @@ __global__ __launch_bounds__(T *(N / Sched<N>::E), MINW) void k_xfwd_rk(const cx
 // update straight into the curl prologue and inverse x transform, saving
 // the re-read of the state.  W fields 0..2 (this stage's N) are read before
 // the same pencils' fields 0..5 are overwritten with the next partials.
-template <int N, typename R, int T, int STAGE, typename SR>
-__global__ __launch_bounds__(T *(N / Sched<N>::E)) void k_xfwd_rk_xinv(cx<R> *__restrict__ W,
+template <int N, typename R, int T, int STAGE, typename SR, int MINW = 1, int EPI = 0, int E = Sched<N>::E>
+__global__ __launch_bounds__(T *(N / E), MINW) void k_xfwd_rk_xinv(cx<R> *__restrict__ W,
                                                                        cx<SR> *__restrict__ uhr,
                                                                        double2 *__restrict__ uhn,
                                                                        double2 *__restrict__ acc,
                                                                        double2 *__restrict__ phat, Geo g,
                                                                        RKParams q, const cx<R> *__restrict__ tw) {
-  constexpr int E = Sched<N>::E;
   __shared__ cx<R> lds[T * padlen<N>()];
+  __shared__ cx<R> twl[tw_size<N>()];
+  tw = stage_twiddles<N, R>(twl, tw);
   const int p = threadIdx.x % T, t = threadIdx.x / T;
   const int j = blockIdx.y, k = blockIdx.x * T + p;
   const bool ok = k < N / 2 + 1;
   cx<R> U[E], V[E], Wv[E];
-  forward_x_rk<N, R, T, STAGE, SR>(W, uhr, uhn, acc, phat, g, q, tw, t, p, j, k, ok, lds, U, V, Wv);
-  curl_inverse_x<N, R, T, (STAGE < 3)>(U, V, Wv, W + blockIdx.z * g.mW, g, t, p, j, k, ok, lds, tw);
+  forward_x_rk<N, R, T, STAGE, SR, EPI, E>(W, uhr, uhn, acc, phat, g, q, tw, t, p, j, k, ok, lds, U, V, Wv);
+  curl_inverse_x<N, R, T, (STAGE < 3), E>(U, V, Wv, W + blockIdx.z * g.mW, g, t, p, j, k, ok, lds, tw);
 }
 
 // ---------------------------------------------------------------------------
