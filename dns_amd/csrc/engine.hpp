@@ -356,13 +356,13 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
         // pass A of this stage ran fused into the previous stage's E
         if (rk == 0) {
           DNS_TIMED(c, sl + 1, (k_ypass<N, R, TY, true, 1, true><<<gy, by, 0, s>>>(W + 3 * g.fs, 3, g, tw, g.mW)));
-          DNS_TIMED(c, sl + 2, (k_zfused<N, R, PP, 0, R, 1, true><<<gz, bz, 0, s>>>(W, (R *)c->us, g, invn3, tw)));
+          DNS_TIMED(c, sl + 2, (k_zfused<N, R, PP, 0, R><<<gz, bz, 0, s>>>(W, (R *)c->us, g, invn3, tw)));
         } else {
           DNS_TIMED(c, sl + 1, (k_ypass<N, R, TY, true, 1, true><<<gy, by, 0, s>>>(W, 6, g, tw, g.mW)));
           if (rk < 3)
-            DNS_TIMED(c, sl + 2, (k_zfused<N, R, PP, 1, R, 1, true><<<gz, bz, 0, s>>>(W, (R *)c->us, g, invn3, tw)));
+            DNS_TIMED(c, sl + 2, (k_zfused<N, R, PP, 1, R><<<gz, bz, 0, s>>>(W, (R *)c->us, g, invn3, tw)));
           else
-            DNS_TIMED(c, sl + 2, (k_zfused<N, R, PP, 2, R, 1, true><<<gz, bz, 0, s>>>(W, (R *)c->us, g, invn3, tw)));
+            DNS_TIMED(c, sl + 2, (k_zfused<N, R, PP, 2, R><<<gz, bz, 0, s>>>(W, (R *)c->us, g, invn3, tw)));
         }
         DNS_TIMED(c, sl + 3,
                   (k_ypass<N, R, TYF, false, 1, PIPE_YF, true><<<gy_trunc, byf, 0, s>>>(W, 3, g, tw, g.mW)));

@@ -265,58 +265,40 @@ __device__ __forceinline__ void split_store(const cx<R> (&f)[Sched<N>::E], cx<R>
   __syncthreads();
 }
 
-// C: fused z pass of one RK stage.
-//  MODE 0 (stage 1): u,v,w from the stale physical velocity `us`,
-//                    omega from W fields 3..5;
-//  MODE 1 (stages 2,3): u,v,w,omega from W fields 0..5;
-//  MODE 2 (stage 4): as MODE 1 and the physical u,v,w are written to `us`.
-// Output: N = u x omega (main.c:374-376) transformed along z into W 0..2.
-// Grid n*n/PPW blocks.
-// PHASED: load and transform one Hermitian pair at a time (lower register
-// pressure, loads not hoisted across the transforms' barriers).
-template <int N, typename R, int PPW, int MODE, typename UR, int MINW = 1, bool PHASED = false>
-__global__ __launch_bounds__(PPW *(N / Sched<N>::E), MINW) void k_zfused(cx<R> *__restrict__ W, UR *__restrict__ us,
-                                                                    Geo g, R invn3,
-                                                                    const cx<R> *__restrict__ tw) {
-  constexpr int E = Sched<N>::E, TPP = N / E, H = N / 2;
-  __shared__ cx<R> lds[PPW * padlen<N>()];
-  const int p = threadIdx.x / TPP, t = threadIdx.x % TPP;
-  const long row = (long)blockIdx.x * PPW + p;
-  us += blockIdx.z * g.mR;
-  cx<R> *w = W + blockIdx.z * g.mW + row * g.nfp;
-  const PencilSlot<N> slot{p * padlen<N>()};
+// One row of the fused z pass: c2r of u and omega (MODE 0: omega_z given in
+// oz0, u from the stale physical velocity), u x omega, forward transform of
+// (N_x, N_y) stored; N_z returned in nz.  Loads one Hermitian pair at a
+// time (register budget).
+template <int N, typename R, int MODE, typename UR>
+__device__ __forceinline__ void zrow(cx<R> *__restrict__ w, UR *__restrict__ usr, Geo g, R invn3,
+                                     const cx<R> *__restrict__ tw, int t, cx<R> *lds, PencilSlot<N> slot,
+                                     const R (&oz0)[Sched<N>::E], R (&nz)[Sched<N>::E]) {
+  constexpr int E = Sched<N>::E, TPP = N / E;
   R u[E], v[E], ww[E], ox[E], oy[E], oz[E];
   if constexpr (MODE == 0) {
-    cx<R> z1[E], z2[E];
+    cx<R> z1[E];
 #pragma unroll
     for (int c = 0; c < E; c++) {
-      const long l = row * N + t + TPP * c;
-      u[c] = (R)us[l];
-      v[c] = (R)us[g.rs + l];
-      ww[c] = (R)us[2 * g.rs + l];
+      const long l = t + TPP * c;
+      u[c] = (R)usr[l];
+      v[c] = (R)usr[g.rs + l];
+      ww[c] = (R)usr[2 * g.rs + l];
     }
     herm_pair<N, R>(z1, w + 3 * g.fs, w + 4 * g.fs, t);
-    if (!PHASED) herm_pair<N, R>(z2, w + 5 * g.fs, nullptr, t);
     fft_pencil<N, R, true>(z1, t, lds, slot, tw);
-    if (PHASED) herm_pair<N, R>(z2, w + 5 * g.fs, nullptr, t);
-    fft_pencil<N, R, true>(z2, t, lds, slot, tw);
 #pragma unroll
     for (int c = 0; c < E; c++) {
       ox[c] = z1[c].x * invn3;
       oy[c] = z1[c].y * invn3;
-      oz[c] = z2[c].x * invn3;
+      oz[c] = oz0[c];
     }
   } else {
     cx<R> z1[E], z2[E], z3[E];
     herm_pair<N, R>(z1, w, w + g.fs, t);
-    if (!PHASED) {
-      herm_pair<N, R>(z2, w + 2 * g.fs, w + 3 * g.fs, t);
-      herm_pair<N, R>(z3, w + 4 * g.fs, w + 5 * g.fs, t);
-    }
     fft_pencil<N, R, true>(z1, t, lds, slot, tw);
-    if (PHASED) herm_pair<N, R>(z2, w + 2 * g.fs, w + 3 * g.fs, t);
+    herm_pair<N, R>(z2, w + 2 * g.fs, w + 3 * g.fs, t);
     fft_pencil<N, R, true>(z2, t, lds, slot, tw);
-    if (PHASED) herm_pair<N, R>(z3, w + 4 * g.fs, w + 5 * g.fs, t);
+    herm_pair<N, R>(z3, w + 4 * g.fs, w + 5 * g.fs, t);
     fft_pencil<N, R, true>(z3, t, lds, slot, tw);
 #pragma unroll
     for (int c = 0; c < E; c++) {
@@ -330,29 +312,64 @@ __global__ __launch_bounds__(PPW *(N / Sched<N>::E), MINW) void k_zfused(cx<R> *
     if constexpr (MODE == 2) {
 #pragma unroll
       for (int c = 0; c < E; c++) {
-        const long l = row * N + t + TPP * c;
-        st_real<UR>(us + l, (UR)u[c]);
-        st_real<UR>(us + g.rs + l, (UR)v[c]);
-        st_real<UR>(us + 2 * g.rs + l, (UR)ww[c]);
+        const long l = t + TPP * c;
+        st_real<UR>(usr + l, (UR)u[c]);
+        st_real<UR>(usr + g.rs + l, (UR)v[c]);
+        st_real<UR>(usr + 2 * g.rs + l, (UR)ww[c]);
       }
     }
   }
-  cx<R> f1[E], f2[E];
+  cx<R> f1[E];
 #pragma unroll
   for (int c = 0; c < E; c++) {
     f1[c] = mk<R>(v[c] * oz[c] - ww[c] * oy[c], ww[c] * ox[c] - u[c] * oz[c]);
-    f2[c] = mk<R>(u[c] * oy[c] - v[c] * ox[c], (R)0);
+    nz[c] = u[c] * oy[c] - v[c] * ox[c];
   }
   fft_pencil<N, R, false>(f1, t, lds, slot, tw);
+  split_store<N, R>(f1, w, w + g.fs, t, lds, slot, g.nk);
+}
+
+// C: fused z pass of one RK stage, one row per wave.
+//  MODE 0 (stage 1): u,v,w from the stale physical velocity `us`,
+//                    omega from W fields 3..5;
+//  MODE 1 (stages 2,3): u,v,w,omega from W fields 0..5;
+//  MODE 2 (stage 4): as MODE 1 and the physical u,v,w are written to `us`.
+// Output: N = u x omega (main.c:374-376) transformed along z into W 0..2,
+// kz < kmax only (the rest is removed by the dealias mask).
+// (N_x, N_y) are transformed and stored before N_z is transformed, so the
+// two forward transforms are never live together (128 instead of 137
+// VGPRs in float: four waves per SIMD instead of three).
+// Grid n*n/PPW blocks.
+template <int N, typename R, int PPW, int MODE, typename UR, int MINW = 1>
+__global__ __launch_bounds__(PPW *(N / Sched<N>::E), MINW) void k_zfused(cx<R> *__restrict__ W, UR *__restrict__ us,
+                                                                    Geo g, R invn3,
+                                                                    const cx<R> *__restrict__ tw) {
+  constexpr int E = Sched<N>::E, TPP = N / E, H = N / 2;
+  __shared__ cx<R> lds[PPW * padlen<N>()];
+  const int p = threadIdx.x / TPP, t = threadIdx.x % TPP;
+  const long row = (long)blockIdx.x * PPW + p;
+  us += blockIdx.z * g.mR;
+  cx<R> *w = W + blockIdx.z * g.mW + row * g.nfp;
+  const PencilSlot<N> slot{p * padlen<N>()};
+  R oz0[E], nz[E];
+  if constexpr (MODE == 0) {
+    cx<R> z[E];
+    herm_pair<N, R>(z, w + 5 * g.fs, nullptr, t);
+    fft_pencil<N, R, true>(z, t, lds, slot, tw);
+#pragma unroll
+    for (int c = 0; c < E; c++) oz0[c] = z[c].x * invn3;
+  }
+  zrow<N, R, MODE, UR>(w, us + row * N, g, invn3, tw, t, lds, slot, oz0, nz);
+  // N_z is the transform of a real pencil: its first half is the r2c output.
+  cx<R> f2[E];
+#pragma unroll
+  for (int c = 0; c < E; c++) f2[c] = mk<R>(nz[c], (R)0);
   fft_pencil<N, R, false>(f2, t, lds, slot, tw);
-  // f2 is the transform of a real pencil: its first half is the r2c output.
-  // Only kz < kmax is kept: the rest is removed by the dealias mask.
 #pragma unroll
   for (int c = 0; c <= E / 2; c++) {
     const int pos = t + TPP * c;
     if (pos <= H && pos < g.nk) st<R>(w + 2 * g.fs + pos, f2[c]);
   }
-  split_store<N, R>(f1, w, w + g.fs, t, lds, slot, g.nk);
 }
 
 // Plain r2c along z of `nfields` physical double fields into spectral
