@@ -36,7 +36,8 @@ static __global__ void k_adj_spec_pre(const double2 *__restrict__ lamU, const do
                                double b, double dt, double nu, double kmax) {
   const int nf = n / 2 + 1;
   const long row = blockIdx.x;
-  const int i = (int)(row / n), j = (int)(row % n);
+  int i, j;
+  state_row(row, n, i, j);
   const double kx = i < n / 2 ? i : i - n, ky = j < n / 2 ? j : j - n;
   for (int k = threadIdx.x; k < nf; k += blockDim.x) {
     const long l = row * g.nfp + k;
@@ -105,7 +106,8 @@ static __global__ void k_adj_spec_post(const double2 *__restrict__ Fw, const dou
                                 double2 *__restrict__ lamUr, double2 *__restrict__ lamUn, int n, Geo g) {
   const int nf = n / 2 + 1;
   const long row = blockIdx.x;
-  const int i = (int)(row / n), j = (int)(row % n);
+  int i, j;
+  state_row(row, n, i, j);
   const double kx = i < n / 2 ? i : i - n, ky = j < n / 2 ? j : j - n;
   for (int k = threadIdx.x; k < nf; k += blockDim.x) {
     const long l = row * g.nfp + k;
@@ -141,7 +143,8 @@ static __global__ void k_adj_spec_post(const double2 *__restrict__ Fw, const dou
 static __global__ void k_curl_spec(const double2 *__restrict__ in, double2 *__restrict__ out, int n, Geo g) {
   const int nf = n / 2 + 1;
   const long row = blockIdx.x;
-  const int i = (int)(row / n), j = (int)(row % n);
+  int i, j;
+  state_row(row, n, i, j);
   const double kx = i < n / 2 ? i : i - n, ky = j < n / 2 ? j : j - n;
   for (int k = threadIdx.x; k < nf; k += blockDim.x) {
     const long l = row * g.nfp + k;
@@ -163,7 +166,8 @@ static __global__ void k_enstrophy_grad(const double2 *__restrict__ U, double2 *
                                  double inv6) {
   const int nf = n / 2 + 1;
   const long row = blockIdx.x;
-  const int i = (int)(row / n), j = (int)(row % n);
+  int i, j;
+  state_row(row, n, i, j);
   const double kx = i < n / 2 ? i : i - n, ky = j < n / 2 ? j : j - n;
   for (int k = threadIdx.x; k < g.nfp; k += blockDim.x) {
     const long l = row * g.nfp + k;

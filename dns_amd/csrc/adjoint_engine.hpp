@@ -58,11 +58,12 @@ template <int N> struct Adjoint {
     return rc;
   }
 
+  // Every spectral array of the adjoint is in the state layout.
   // phys[f] = scale * C2R(spec[f]), f < 3; scr: 3 spectral fields of scratch.
   static int c2r3(dns_ctx *c, const double2 *spec, double *phys, double scale, double2 *scr) {
     hipStream_t s = c->stream;
     const cx<double> *tw = (const cx<double> *)c->tw64;
-    k_xplain<N, double, T, true, double><<<gx(), T * TPP, 0, s>>>(spec, scr, 3, c->g, tw);
+    k_xplain<N, double, T, true, double, true><<<gx(), T * TPP, 0, s>>>(spec, scr, 3, c->g, tw);
     DNS_LAUNCHED();
     k_ypass<N, double, T, true><<<gx(), T * TPP, 0, s>>>(scr, 3, c->g, tw);
     DNS_LAUNCHED();
@@ -74,15 +75,15 @@ template <int N> struct Adjoint {
     return DNS_OK;
   }
 
-  // spec[f] = R2C(phys[f]), f < 3.
-  static int r2c3(dns_ctx *c, const double *phys, double2 *spec) {
+  // spec[f] = R2C(phys[f]), f < 3; scr: 3 spectral fields of scratch.
+  static int r2c3(dns_ctx *c, const double *phys, double2 *spec, double2 *scr) {
     hipStream_t s = c->stream;
     const cx<double> *tw = (const cx<double> *)c->tw64;
-    k_zr2c<N, double, PPW><<<N * N / PPW, PPW * TPP, 0, s>>>(phys, spec, 3, c->g, tw);
+    k_zr2c<N, double, PPW><<<N * N / PPW, PPW * TPP, 0, s>>>(phys, scr, 3, c->g, tw);
     DNS_LAUNCHED();
-    k_ypass<N, double, T, false><<<gx(), T * TPP, 0, s>>>(spec, 3, c->g, tw);
+    k_ypass<N, double, T, false><<<gx(), T * TPP, 0, s>>>(scr, 3, c->g, tw);
     DNS_LAUNCHED();
-    k_xplain<N, double, T, false, double><<<gx(), T * TPP, 0, s>>>(spec, spec, 3, c->g, tw);
+    k_xplain<N, double, T, false, double, false, true><<<gx(), T * TPP, 0, s>>>(scr, spec, 3, c->g, tw);
     DNS_LAUNCHED();
     return DNS_OK;
   }
@@ -137,8 +138,8 @@ template <int N> struct Adjoint {
       k_adj_cross<<<gp, bp, 0, s>>>(w->lamN, ur, w->w, r == 3 ? w->lamS : nullptr, w->lu, w->lw, g.rs,
                                     r == 0 ? 1.0 : sc, sc);
       DNS_LAUNCHED();
-      if ((rc = r2c3(c, w->lw, w->Fw))) return rc;
-      if (r > 0 && (rc = r2c3(c, w->lu, w->Fu))) return rc;
+      if ((rc = r2c3(c, w->lw, w->Fw, w->scr))) return rc;
+      if (r > 0 && (rc = r2c3(c, w->lu, w->Fu, w->scr))) return rc;
       k_adj_spec_post<<<grow, brow, 0, s>>>(w->Fw, r > 0 ? w->Fu : nullptr, w->lamUr, w->lamUn, N, g);
       DNS_LAUNCHED();
       if (r == 0 && (rc = copy(c, w->lamSout, w->lu, w->R3))) return rc;

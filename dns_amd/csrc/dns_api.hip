@@ -166,21 +166,12 @@ static int ctx_diagnostics(dns_ctx *c, int m, double *energy, double *enstrophy)
 
 static int ctx_get_spectral(dns_ctx *c, int m, double *uvw_hat) {
   DNS_HIP(hipSetDevice(c->device));
-  const double2 *src = (const double2 *)c->uhn + m * c->g.mS;
-  DNS_HIP(hipMemcpy2DAsync(uvw_hat, sizeof(double2) * c->nf, src, sizeof(double2) * c->nfp,
-                           sizeof(double2) * c->nf, (size_t)3 * c->n * c->n, hipMemcpyDeviceToHost, c->stream));
-  DNS_HIP(hipStreamSynchronize(c->stream));
-  return DNS_OK;
+  return c->impl->get_spectral(c, m, uvw_hat);
 }
 
 static int ctx_set_spectral(dns_ctx *c, int m, const double *uvw_hat) {
   DNS_HIP(hipSetDevice(c->device));
-  c->w_ready = 0;
-  double2 *dst = (double2 *)c->uhn + m * c->g.mS;
-  DNS_HIP(hipMemcpy2DAsync(dst, sizeof(double2) * c->nfp, uvw_hat, sizeof(double2) * c->nf,
-                           sizeof(double2) * c->nf, (size_t)3 * c->n * c->n, hipMemcpyHostToDevice, c->stream));
-  DNS_HIP(hipStreamSynchronize(c->stream));
-  return DNS_OK;
+  return c->impl->set_spectral(c, m, uvw_hat);
 }
 
 // Time every launch shape of the context for `steps` steps from its

@@ -108,6 +108,9 @@ struct Impl {
   virtual int irfft3(dns_ctx *c, const double *in, double *out) = 0;
   virtual int get_us(dns_ctx *c, int m, double *out) = 0;
   virtual int set_us(dns_ctx *c, int m, const double *in) = 0;
+  // spectral state of member m <-> host [3][n][n][n/2+1] complex (x-major)
+  virtual int get_spectral(dns_ctx *c, int m, double *out) = 0;
+  virtual int set_spectral(dns_ctx *c, int m, const double *in) = 0;
   // discrete adjoint (FP64 contexts, member 0)
   virtual int adj_forward(dns_ctx *c, const double *u0, long nsteps, long every) = 0;
   virtual int adj_backward(dns_ctx *c, double seed, double *grad) = 0;
@@ -227,27 +230,12 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
     return DNS_OK;
   }
 
-  // Forward 3-D transform (double) of `nfields` physical fields at `phys`
-  // into spectral fields at `spec` (padded layout), one member.
-  int fwd3_double(dns_ctx *c, const double *phys, double2 *spec, int nfields) {
-    hipStream_t s = c->stream;
-    k_zr2c<N, double, PPW><<<N * N / PPW, PPW * TPP, 0, s>>>(phys, spec, nfields, c->g, twD(c));
-    DNS_LAUNCHED();
-    k_ypass<N, double, T_D, false><<<grid_x(T_D), T_D * TPP, 0, s>>>(spec, nfields, c->g, twD(c));
-    DNS_LAUNCHED();
-    k_xplain<N, double, T_D, false, double><<<grid_x(T_D), T_D * TPP, 0, s>>>(spec, spec, nfields, c->g, twD(c));
-    DNS_LAUNCHED();
-    return DNS_OK;
-  }
-
   int load(dns_ctx *c, int m, const double *uvw) override {
     c->w_ready = 0;
     hipStream_t s = c->stream;
     double *tmp = (double *)wslice(c, m);  // 3 physical double fields fit in a member's W
     const size_t rb = sizeof(double) * c->g.rs;
     DNS_HIP(hipMemcpyAsync(tmp, uvw, 3 * rb, hipMemcpyDefault, s));
-    int rc = fwd3_double(c, tmp, (double2 *)c->uhn + m * c->g.mS, 3);
-    if (rc) return rc;
     R *us = (R *)c->us + m * c->g.mR;
     if (sizeof(R) == 8) {
       DNS_HIP(hipMemcpyAsync(us, tmp, 3 * rb, hipMemcpyDeviceToDevice, s));
@@ -255,6 +243,43 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
       k_convert<double, R><<<4096, 256, 0, s>>>(tmp, us, 3 * c->g.rs);
       DNS_LAUNCHED();
     }
+    // partials in the member's uhn slice, result in its W slice (the
+    // physical input is dead by then), then copied into uhn
+    double2 *uh = (double2 *)c->uhn + m * c->g.mS;
+    double2 *ws = (double2 *)wslice(c, m);
+    k_zr2c<N, double, PPW><<<N * N / PPW, PPW * TPP, 0, s>>>(tmp, uh, 3, c->g, twD(c));
+    DNS_LAUNCHED();
+    k_ypass<N, double, T_D, false><<<grid_x(T_D), T_D * TPP, 0, s>>>(uh, 3, c->g, twD(c));
+    DNS_LAUNCHED();
+    k_xplain<N, double, T_D, false, double, false, true><<<grid_x(T_D), T_D * TPP, 0, s>>>(uh, ws, 3, c->g, twD(c));
+    DNS_LAUNCHED();
+    DNS_HIP(hipMemcpyAsync(uh, ws, 3 * sizeof(double2) * c->g.fs, hipMemcpyDeviceToDevice, s));
+    DNS_HIP(hipStreamSynchronize(s));
+    return DNS_OK;
+  }
+
+  int get_spectral(dns_ctx *c, int m, double *out) override {
+    c->w_ready = 0;
+    hipStream_t s = c->stream;
+    double2 *ws = (double2 *)wslice(c, m);
+    k_swap_rows<double2><<<N * N, 256, 0, s>>>((const double2 *)c->uhn + m * c->g.mS, ws, N, c->g, 3);
+    DNS_LAUNCHED();
+    DNS_HIP(hipMemcpy2DAsync(out, sizeof(double2) * NF, ws, sizeof(double2) * c->g.nfp, sizeof(double2) * NF,
+                             (size_t)3 * N * N, hipMemcpyDeviceToHost, s));
+    DNS_HIP(hipStreamSynchronize(s));
+    return DNS_OK;
+  }
+
+  int set_spectral(dns_ctx *c, int m, const double *in) override {
+    c->w_ready = 0;
+    hipStream_t s = c->stream;
+    double2 *ws = (double2 *)wslice(c, m);
+    const size_t pitch = sizeof(double2) * c->g.nfp;
+    DNS_HIP(hipMemset2DAsync(ws, pitch, 0, pitch, (size_t)3 * N * N, s));
+    DNS_HIP(hipMemcpy2DAsync(ws, pitch, in, sizeof(double2) * NF, sizeof(double2) * NF, (size_t)3 * N * N,
+                             hipMemcpyHostToDevice, s));
+    k_swap_rows<double2><<<N * N, 256, 0, s>>>(ws, (double2 *)c->uhn + m * c->g.mS, N, c->g, 3);
+    DNS_LAUNCHED();
     DNS_HIP(hipStreamSynchronize(s));
     return DNS_OK;
   }
@@ -392,9 +417,10 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
 
   // c2r (double) of one spectral double field into a physical double field
   // times `scale`, one member.
+  // c2r of one spectral state field (state layout) into `phys`, x scale.
   int inv3_double(dns_ctx *c, const double2 *spec, double2 *tmpc, double *phys, double scale) {
     hipStream_t s = c->stream;
-    k_xplain<N, double, T_D, true, double><<<grid_x(T_D), T_D * TPP, 0, s>>>(spec, tmpc, 1, c->g, twD(c));
+    k_xplain<N, double, T_D, true, double, true><<<grid_x(T_D), T_D * TPP, 0, s>>>(spec, tmpc, 1, c->g, twD(c));
     DNS_LAUNCHED();
     k_ypass<N, double, T_D, true><<<grid_x(T_D), T_D * TPP, 0, s>>>(tmpc, 1, c->g, twD(c));
     DNS_LAUNCHED();
@@ -539,15 +565,20 @@ int Engine<N, R, SR>::adj_step(dns_ctx *c, const double *uh, const double *us, c
     if (rc) return rc;
     hipStream_t s = c->stream;
     const size_t pitch = sizeof(double2) * c->nfp, width = sizeof(double2) * NF, rows = (size_t)3 * N * N;
-    // step-start state in w->scr / w->lu (scratch until backward_step reuses
-    // them: it copies both into the context first)
+    // step-start state in w->Fw / w->lw (scratch until backward_step reuses
+    // them: it copies both into the context first); host spectra are
+    // x-major and go through tmpC / tmpN into the state layout
     double2 *U = w->Fw;
     double *u = w->lw;
-    DNS_HIP(hipMemset2DAsync(U, pitch, 0, pitch, rows, s));
-    DNS_HIP(hipMemcpy2DAsync(U, pitch, uh, width, width, rows, hipMemcpyDefault, s));
+    DNS_HIP(hipMemset2DAsync(w->tmpC, pitch, 0, pitch, rows, s));
+    DNS_HIP(hipMemcpy2DAsync(w->tmpC, pitch, uh, width, width, rows, hipMemcpyDefault, s));
+    k_swap_rows<double2><<<N * N, 256, 0, s>>>(w->tmpC, U, N, c->g, 3);
+    DNS_LAUNCHED();
     DNS_HIP(hipMemcpyAsync(u, us, w->R3, hipMemcpyDefault, s));
-    DNS_HIP(hipMemset2DAsync(w->lamU, pitch, 0, pitch, rows, s));
-    DNS_HIP(hipMemcpy2DAsync(w->lamU, pitch, lam_uh, width, width, rows, hipMemcpyDefault, s));
+    DNS_HIP(hipMemset2DAsync(w->tmpN, pitch, 0, pitch, rows, s));
+    DNS_HIP(hipMemcpy2DAsync(w->tmpN, pitch, lam_uh, width, width, rows, hipMemcpyDefault, s));
+    k_swap_rows<double2><<<N * N, 256, 0, s>>>(w->tmpN, w->lamU, N, c->g, 3);
+    DNS_LAUNCHED();
     DNS_HIP(hipMemcpyAsync(w->lamS, lam_us, w->R3, hipMemcpyDefault, s));
     // backward_step overwrites Fw and lw; keep the inputs in the segment store
     void *st = w->seg.empty() ? Adjoint<N>::state_alloc(c, w) : w->seg[0];
@@ -557,7 +588,9 @@ int Engine<N, R, SR>::adj_step(dns_ctx *c, const double *uh, const double *us, c
     DNS_HIP(hipMemcpyAsync((char *)st + w->C3, u, w->R3, hipMemcpyDeviceToDevice, s));
     rc = Adjoint<N>::backward_step(c, w, (const double2 *)st, (const double *)((char *)st + w->C3));
     if (rc) return rc;
-    DNS_HIP(hipMemcpy2DAsync(out_uh, width, w->lamU, pitch, width, rows, hipMemcpyDefault, s));
+    k_swap_rows<double2><<<N * N, 256, 0, s>>>(w->lamU, w->tmpN, N, c->g, 3);
+    DNS_LAUNCHED();
+    DNS_HIP(hipMemcpy2DAsync(out_uh, width, w->tmpN, pitch, width, rows, hipMemcpyDefault, s));
     DNS_HIP(hipMemcpyAsync(out_us, w->lamS, w->R3, hipMemcpyDefault, s));
     DNS_HIP(hipStreamSynchronize(s));
     c->w_ready = 0;

@@ -1,9 +1,15 @@
 // HIP kernels of the RK4 step (gfx950).  All arrays are device-resident.
 //
-// Layouts (i = x slowest, j = y, k = z fastest, as in main.c:236,358):
-//   spectral field  : complex [n][n][nfp], element (i,j,k) at (i*n+j)*nfp+k,
+// Layouts (i = x, j = y, k = z fastest):
+//   spectral partials (W): complex [n][n][nfp], element (i,j,k) at
+//                     (i*n+j)*nfp+k (x slowest, as main.c:236,358),
 //                     k < nf = n/2+1 valid, nfp = nf rounded up to 16
 //                     (rows start on 128-byte boundaries);
+//   spectral state (uhn, uhr, acc, phat): complex [n][n][nfp] with y
+//                     slowest, element (i,j,k) at (j*n+i)*nfp+k: only the
+//                     x pass touches the state during a step, and in this
+//                     layout its x-pencils have the short row stride of a
+//                     y pass (float x pass 2.72 -> 2.39 ms at n=512);
 //   physical field  : real [n][n][n], (i*n+j)*n+k.
 // A "pencil" is the set of elements along one axis with the other two
 // indices fixed.  x and y pencils are strided: a block owns T pencils with
@@ -56,6 +62,24 @@ template <typename RO, typename RI> __device__ __forceinline__ cx<RO> cvt(cx<RI>
 // otherwise (main.c:224-230, Nyquist = -n/2).
 template <int N> __device__ __forceinline__ int wave(int i) { return i < N / 2 ? i : i - N; }
 
+// Row r of a spectral state field holds (i, j) = (r % n, r / n).
+__device__ __forceinline__ void state_row(long r, int n, int &i, int &j) {
+  i = (int)(r % n);
+  j = (int)(r / n);
+}
+
+// Swaps the two row indices of `nfields` padded spectral fields:
+// dst[(b*n+a)*nfp+k] = src[(a*n+b)*nfp+k] (x-major <-> state layout; the
+// map is its own inverse).  Grid (n*n) blocks.
+template <typename T>
+__global__ __launch_bounds__(256) void k_swap_rows(const T *__restrict__ src, T *__restrict__ dst, int n, Geo g,
+                                                   int nfields) {
+  const long r = blockIdx.x, a = r / n, b = r % n;
+  for (int f = 0; f < nfields; f++)
+    for (int k = threadIdx.x; k < g.nfp; k += blockDim.x)
+      dst[f * g.fs + (b * n + a) * g.nfp + k] = src[f * g.fs + r * g.nfp + k];
+}
+
 // ---------------------------------------------------------------------------
 // B/D: FFT along y of `nfields` consecutive spectral fields, in place.
 // Grid (ceil(nf/T), n): blockIdx.y = i.
@@ -104,9 +128,10 @@ __global__ __launch_bounds__(T *(N / E), MINW) void k_ypass(cx<R> *__restrict__ 
 }
 
 // Plain FFT along x of `nfields` fields: in (precision RI) -> out (R).
-// Used for the initial transform and for snapshots; in may equal out.
-// Grid (ceil(nf/T), n): blockIdx.y = j.
-template <int N, typename R, int T, bool INV, typename RI>
+// Used for the initial transform and for snapshots.  SIN / SOUT: the input
+// / output is in the state layout; in may equal out only when both layouts
+// are the same.  Grid (ceil(nf/T), n): blockIdx.y = j.
+template <int N, typename R, int T, bool INV, typename RI, bool SIN = false, bool SOUT = false>
 __global__ __launch_bounds__(T *(N / Sched<N>::E)) void k_xplain(const cx<RI> *in, cx<R> *out, int nfields, Geo g,
                                                                   const cx<R> *__restrict__ tw) {
   constexpr int E = Sched<N>::E, TPP = N / E;
@@ -117,15 +142,18 @@ __global__ __launch_bounds__(T *(N / Sched<N>::E)) void k_xplain(const cx<RI> *i
   const int k = blockIdx.x * T + p;
   const bool ok = k < N / 2 + 1;
   const long off = (long)blockIdx.y * g.nfp + k, istr = (long)N * g.nfp;
+  const long soff = (long)blockIdx.y * N * g.nfp + k, sstr = g.nfp;
+  const long ioff = SIN ? soff : off, iistr = SIN ? sstr : istr;
+  const long ooff = SOUT ? soff : off, oistr = SOUT ? sstr : istr;
   for (int f = 0; f < nfields; f++) {
     cx<R> v[E];
 #pragma unroll
     for (int c = 0; c < E; c++)
-      v[c] = ok ? cvt<R, RI>(in[f * g.fs + off + (t + TPP * c) * istr]) : zero<R>();
+      v[c] = ok ? cvt<R, RI>(in[f * g.fs + ioff + (t + TPP * c) * iistr]) : zero<R>();
     fft_pencil<N, R, INV>(v, t, lds, TileSlot<N, T>{p}, tw);
     if (ok) {
 #pragma unroll
-      for (int c = 0; c < E; c++) st<R>(out + f * g.fs + off + (t + TPP * c) * istr, v[c]);
+      for (int c = 0; c < E; c++) st<R>(out + f * g.fs + ooff + (t + TPP * c) * oistr, v[c]);
     }
   }
 }
@@ -152,45 +180,35 @@ __device__ __forceinline__ void curl_inverse_x(cx<R> (&U)[E], cx<R> (&V)[E], cx<
       for (int c = 0; c < E; c++) st<R>(W + f * g.fs + off + (t + TPP * c) * istr, v[c]);
     }
   };
+  auto irot = [](R a, R b) { return mk<R>(-b, a); };  // I*(a + i b)
   {
     cx<R> v[E];
 #pragma unroll
-    for (int c = 0; c < E; c++) {  // curlX = I*(ky W - kz V)
-      cx<R> z = mk<R>(ky * Wv[c].x - kz * V[c].x, ky * Wv[c].y - kz * V[c].y);
-      v[c] = mk<R>(-z.y, z.x);
-    }
+    for (int c = 0; c < E; c++) v[c] = irot(ky * Wv[c].x - kz * V[c].x, ky * Wv[c].y - kz * V[c].y);  // curlX
     fft_pencil<N, R, true>(v, t, lds, slot, tw);
     store(3, v);
-  }
-  {
-    cx<R> v[E];
 #pragma unroll
     for (int c = 0; c < E; c++) {  // curlY = I*(kz U - kx W)
       const R kx = (R)wave<N>(t + TPP * c);
-      cx<R> z = mk<R>(kz * U[c].x - kx * Wv[c].x, kz * U[c].y - kx * Wv[c].y);
-      v[c] = mk<R>(-z.y, z.x);
+      v[c] = irot(kz * U[c].x - kx * Wv[c].x, kz * U[c].y - kx * Wv[c].y);
     }
     fft_pencil<N, R, true>(v, t, lds, slot, tw);
     store(4, v);
-  }
-  {
-    cx<R> v[E];
 #pragma unroll
     for (int c = 0; c < E; c++) {  // curlZ = I*(kx V - ky U)
       const R kx = (R)wave<N>(t + TPP * c);
-      cx<R> z = mk<R>(kx * V[c].x - ky * U[c].x, kx * V[c].y - ky * U[c].y);
-      v[c] = mk<R>(-z.y, z.x);
+      v[c] = irot(kx * V[c].x - ky * U[c].x, kx * V[c].y - ky * U[c].y);
     }
     fft_pencil<N, R, true>(v, t, lds, slot, tw);
     store(5, v);
-  }
-  if constexpr (WITH_U) {
-    fft_pencil<N, R, true>(U, t, lds, slot, tw);
-    store(0, U);
-    fft_pencil<N, R, true>(V, t, lds, slot, tw);
-    store(1, V);
-    fft_pencil<N, R, true>(Wv, t, lds, slot, tw);
-    store(2, Wv);
+    if constexpr (WITH_U) {
+      fft_pencil<N, R, true>(U, t, lds, slot, tw);
+      store(0, U);
+      fft_pencil<N, R, true>(V, t, lds, slot, tw);
+      store(1, V);
+      fft_pencil<N, R, true>(Wv, t, lds, slot, tw);
+      store(2, Wv);
+    }
   }
 }
 
@@ -205,13 +223,13 @@ __global__ __launch_bounds__(T *(N / E), MINW) void k_xinv_curl(const cx<SI> *__
   const int p = threadIdx.x % T, t = threadIdx.x / T;
   const int j = blockIdx.y, k = blockIdx.x * T + p;
   const bool ok = k < N / 2 + 1;
-  const long off = (long)j * g.nfp + k, istr = (long)N * g.nfp;
   uh += blockIdx.z * g.mS;
   W += blockIdx.z * g.mW;
+  const long soff = (long)j * N * g.nfp + k;  // state layout
   cx<R> U[E], V[E], Wv[E];
 #pragma unroll
   for (int c = 0; c < E; c++) {
-    const long l = off + (t + TPP * c) * istr;
+    const long l = soff + (t + TPP * c) * g.nfp;
     U[c] = ok ? cvt<R, SI>(uh[l]) : zero<R>();
     V[c] = ok ? cvt<R, SI>(uh[g.fs + l]) : zero<R>();
     Wv[c] = ok ? cvt<R, SI>(uh[2 * g.fs + l]) : zero<R>();
@@ -451,6 +469,8 @@ __device__ __forceinline__ void forward_x_rk(const cx<R> *__restrict__ W, cx<SR>
 #pragma clang fp contract(off)
   constexpr int TPP = N / E;
   const long off = (long)j * g.nfp + k, istr = (long)N * g.nfp;
+  // state element (i, j, k) at soff + i * sstr (state layout)
+  const long soff = (long)j * N * g.nfp + k, sstr = g.nfp;
   const TileSlot<N, T> slot{p};
   constexpr bool MIXED_STATE = sizeof(SR) == 4;
   // two positions per batch (six or nine loads; one position for the
@@ -512,7 +532,7 @@ __device__ __forceinline__ void forward_x_rk(const cx<R> *__restrict__ W, cx<SR>
     for (int i = 0; i < BC; i++) {
 #pragma unroll
       for (int f = 0; f < 3; f++) {
-        const long lf = f * g.fs + off + (t + TPP * (c0 + i)) * istr;
+        const long lf = f * g.fs + soff + (t + TPP * (c0 + i)) * sstr;
         if constexpr (STAGE == 0) B.ur[f][i] = uhn[lf];
         else B.ur[f][i] = uhr[lf];
         if constexpr (MIXED_STATE) {
@@ -536,7 +556,7 @@ __device__ __forceinline__ void forward_x_rk(const cx<R> *__restrict__ W, cx<SR>
       const int c = c0 + i;
       U[c] = V[c] = Wv[c] = zero<R>();
       if (!ok) continue;
-      const long l = off + (t + TPP * c) * istr;
+      const long l = soff + (t + TPP * c) * sstr;
       const double kx = (double)wave<N>(t + TPP * c);
       const double kk = kx * kx + ky * ky + kz * kz;
       const double fac = (myz && fabs(kx) < q.kmax) ? dt : 0.0;
@@ -689,7 +709,8 @@ __global__ __launch_bounds__(BS) void k_diag(const double2 *__restrict__ uh, int
   part += 4L * blockIdx.z * gridDim.x;
   double es = 0, ec = 0, zs = 0, zc = 0;
   for (long row = blockIdx.x; row < (long)n * n; row += gridDim.x) {
-    const int i = (int)(row / n), j = (int)(row % n);
+    int i, j;
+    state_row(row, n, i, j);
     const double kx = i < n / 2 ? i : i - n, ky = j < n / 2 ? j : j - n;
     const double2 *r = uh + row * g.nfp;
     for (int k = threadIdx.x; k < nf; k += BS) {
