@@ -16,7 +16,8 @@ __all__ = ["Solver", "Batch", "DnsError", "lib", "LIB_PATH", "FP64", "MIXED", "a
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
-LIB_PATH = os.path.join(PKG, "libdns_hip.so")
+# DNS_LIB selects another build of the library (e.g. to compare two builds)
+LIB_PATH = os.environ.get("DNS_LIB") or os.path.join(PKG, "libdns_hip.so")
 HEADER = os.path.join(ROOT, "include", "dns.h")
 FP64, MIXED = 0, 1
 KERNEL_SLOTS = 24

@@ -180,18 +180,23 @@ namespace dnsk {
 
 template <int N, typename R, typename SR> struct Engine final : Impl {
   static constexpr int E = Sched<N>::E, TPP = N / E;
-  // Column tiles of the strided passes; every block has at most 512
-  // threads (256 registers per thread, no spills at n=1024).
-  //  y passes and double x passes: 8 columns (128-byte row segments in
-  //  double, 64 in float);
-  //  float x passes (stages 1-3): 16 columns (128 bytes) with 16 elements
-  //  per thread where the schedule allows (powers of two from 256 on).
+  // Column tiles of the strided passes: 128-byte row segments (8 double /
+  // 16 float columns).  Blocks holding double state or the fused x pass
+  // have at most 512 threads (256 registers per thread, no spills at
+  // n=1024); the float x pass gets its 16 columns in 512 threads with 16
+  // elements per thread where the schedule allows (powers of two from 256
+  // on); float y passes use up to 1024 threads.
   static constexpr int T_D = 8 * TPP > 512 ? 512 / TPP : 8;
-  static constexpr int T_Y = T_D;
+  // inverse y pass: 8 double / 16 float columns (128-byte row segments;
+  // narrower float tiles over-fetch: 64-byte segments read 1.8x their bytes)
+  static constexpr int T_Y = sizeof(R) == 8 ? T_D : (16 * TPP > 1024 ? 1024 / TPP : 16);
   // forward (truncated) y pass in float: 16 columns, up to 1024 threads
   // (its register needs are small enough)
   static constexpr int T_YF = sizeof(R) == 8 ? T_D : (16 * TPP > 1024 ? 1024 / TPP : 16);
-  static constexpr bool PIPE_YF = sizeof(R) == 8;
+  static constexpr int PIPE_YF = sizeof(R) == 8 ? 1 : 0;
+  // inverse y pass: fields loaded ahead (two in float: 1.47 -> 1.38 ms at
+  // n=512; one in double, where a second costs occupancy)
+  static constexpr int PIPE_Y = sizeof(R) == 8 ? 1 : 2;
   static constexpr int E_X = (sizeof(R) == 4 && E == 8 && N >= 256) ? 16 : E;
   static constexpr int TPP_X = N / E_X;
   static constexpr int T_X = sizeof(R) == 8 ? T_D : (16 * TPP_X > 512 ? 512 / TPP_X : 16);
@@ -202,8 +207,9 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
 
   size_t sr_size() const override { return sizeof(cx<SR>); }
   // bytes per member of the accumulator region: 3 double fields, or in
-  // mixed precision 12 float fields (Un copy and the increments dU0..dU2)
-  size_t acc_bytes(long fs) const override { return sizeof(SR) == 4 ? 12 * sizeof(float2) * fs : 3 * sizeof(double2) * fs; }
+  // mixed precision 6 float fields (float copy of Un and the running sum of
+  // the increments)
+  size_t acc_bytes(long fs) const override { return sizeof(SR) == 4 ? 6 * sizeof(float2) * fs : 3 * sizeof(double2) * fs; }
   size_t r_size() const override { return sizeof(R); }
 
   static const cx<R> *twR(dns_ctx *c) { return (const cx<R> *)(sizeof(R) == 8 ? c->tw64 : c->tw32); }
@@ -355,9 +361,7 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
   }
 
   // TY: inverse y-pass tile; TYF: forward y-pass tile; TX: x-pass tile
-  // (E_X elements per thread for stages 1-3; stage 4 uses the schedule's E
-  // and the y-pass tile, which fits the register budget of its five state
-  // loads per element); PP: z pencils per block.
+  // (E_X elements per thread); PP: z pencils per block.
   template <int TY, int TYF, int TX, int PP> int step_body(dns_ctx *c, long nsteps) {
     hipStream_t s = c->stream;
     const Geo g = c->g;
@@ -380,10 +384,10 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
         const int sl = DNS_NPASS * rk;
         // pass A of this stage ran fused into the previous stage's E
         if (rk == 0) {
-          DNS_TIMED(c, sl + 1, (k_ypass<N, R, TY, true, 1, true><<<gy, by, 0, s>>>(W + 3 * g.fs, 3, g, tw, g.mW)));
+          DNS_TIMED(c, sl + 1, (k_ypass<N, R, TY, true, 1, PIPE_Y><<<gy, by, 0, s>>>(W + 3 * g.fs, 3, g, tw, g.mW)));
           DNS_TIMED(c, sl + 2, (k_zfused<N, R, PP, 0, R><<<gz, bz, 0, s>>>(W, (R *)c->us, g, invn3, tw)));
         } else {
-          DNS_TIMED(c, sl + 1, (k_ypass<N, R, TY, true, 1, true><<<gy, by, 0, s>>>(W, 6, g, tw, g.mW)));
+          DNS_TIMED(c, sl + 1, (k_ypass<N, R, TY, true, 1, PIPE_Y><<<gy, by, 0, s>>>(W, 6, g, tw, g.mW)));
           if (rk < 3)
             DNS_TIMED(c, sl + 2, (k_zfused<N, R, PP, 1, R><<<gz, bz, 0, s>>>(W, (R *)c->us, g, invn3, tw)));
           else
@@ -403,7 +407,7 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
             DNS_TIMED(c, sl + 5, (k_xfwd_rk_xinv<N, R, TX, 2, SR, 1, 0, E_X><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, nullptr, g, q, tw)));
             break;
           default:
-            DNS_TIMED(c, sl + 5, (k_xfwd_rk_xinv<N, R, TY, 3, SR><<<gy, by, 0, s>>>(W, uhr, uhn, acc, ph, g, q, tw)));
+            DNS_TIMED(c, sl + 5, (k_xfwd_rk_xinv<N, R, TX, 3, SR, 1, 0, E_X><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, ph, g, q, tw)));
             break;
         }
         if (rk < 3 && c->rec[rk])

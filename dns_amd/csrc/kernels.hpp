@@ -84,11 +84,12 @@ __global__ __launch_bounds__(256) void k_swap_rows(const T *__restrict__ src, T 
 // B/D: FFT along y of `nfields` consecutive spectral fields, in place.
 // Grid (ceil(nf/T), n): blockIdx.y = i.
 // MINW: minimum resident waves per SIMD the register allocation must allow.
-// PIPE: load field f+1 while field f is transformed.
+// PIPE: number of fields loaded ahead (0: none; 1: field f+1 is loaded
+// while field f is transformed; 2: fields f+1 and f+2).
 // TRUNC (forward transform of the nonlinear term only): the columns
 // k >= g.nk are not read and only rows inside the dealiased box are written;
 // the grid then covers ceil(nk/T) column tiles.
-template <int N, typename R, int T, bool INV, int MINW = 1, bool PIPE = false, bool TRUNC = false,
+template <int N, typename R, int T, bool INV, int MINW = 1, int PIPE = 0, bool TRUNC = false,
           int E = Sched<N>::E>
 __global__ __launch_bounds__(T *(N / E), MINW) void k_ypass(cx<R> *__restrict__ data, int nfields, Geo g,
                                                            const cx<R> *__restrict__ tw, long mstride = 0) {
@@ -105,10 +106,12 @@ __global__ __launch_bounds__(T *(N / E), MINW) void k_ypass(cx<R> *__restrict__ 
 #pragma unroll
     for (int c = 0; c < E; c++) v[c] = ok ? d[(long)(t + TPP * c) * g.nfp] : zero<R>();
   };
-  cx<R> v[E], nx[E];
+  cx<R> v[E], nx[E], nx2[E];
   load(v, 0);
+  if (PIPE >= 2 && nfields > 1) load(nx, 1);
   for (int f = 0; f < nfields; f++) {
-    if (PIPE && f + 1 < nfields) load(nx, f + 1);
+    if (PIPE == 1 && f + 1 < nfields) load(nx, f + 1);
+    if (PIPE == 2 && f + 2 < nfields) load(nx2, f + 2);
     fft_pencil<N, R, INV>(v, t, lds, TileSlot<N, T>{p}, tw);
     if (ok) {
       cx<R> *d = base + f * g.fs;
@@ -119,7 +122,10 @@ __global__ __launch_bounds__(T *(N / E), MINW) void k_ypass(cx<R> *__restrict__ 
     if (f + 1 < nfields) {
       if (PIPE) {
 #pragma unroll
-        for (int c = 0; c < E; c++) v[c] = nx[c];
+        for (int c = 0; c < E; c++) {
+          v[c] = nx[c];
+          if (PIPE == 2) nx[c] = nx2[c];
+        }
       } else {
         load(v, f + 1);
       }
@@ -243,20 +249,39 @@ __global__ __launch_bounds__(T *(N / E), MINW) void k_xinv_curl(const cx<SI> *__
 
 // Complex pencil Z = X + iY of two half-spectrum rows extended to length N
 // by Hermitian symmetry; the imaginary parts of k=0 and k=N/2 are dropped
-// (FFTW c2r semantics).  Y == nullptr means Y = 0.
+// (FFTW c2r semantics).  Y == nullptr means Y = 0.  herm_load issues the
+// loads (so they can be started early), herm_combine builds the pencil.
+template <int N, typename R> struct HermRaw {
+  cx<R> x[Sched<N>::E], y[Sched<N>::E];
+};
 template <int N, typename R>
-__device__ __forceinline__ void herm_pair(cx<R> (&z)[Sched<N>::E], const cx<R> *X, const cx<R> *Y, int t) {
+__device__ __forceinline__ void herm_load(HermRaw<N, R> &h, const cx<R> *X, const cx<R> *Y, int t) {
   constexpr int E = Sched<N>::E, TPP = N / E, H = N / 2;
 #pragma unroll
   for (int c = 0; c < E; c++) {
     const int pos = t + TPP * c;
     const int k = pos <= H ? pos : N - pos;
-    const cx<R> x = X[k];
-    const cx<R> y = Y ? Y[k] : zero<R>();
+    h.x[c] = X[k];
+    h.y[c] = Y ? Y[k] : zero<R>();
+  }
+}
+template <int N, typename R>
+__device__ __forceinline__ void herm_combine(cx<R> (&z)[Sched<N>::E], const HermRaw<N, R> &h, int t) {
+  constexpr int E = Sched<N>::E, TPP = N / E, H = N / 2;
+#pragma unroll
+  for (int c = 0; c < E; c++) {
+    const int pos = t + TPP * c;
+    const cx<R> x = h.x[c], y = h.y[c];
     if (pos == 0 || pos == H) z[c] = mk<R>(x.x, y.x);
     else if (pos < H) z[c] = mk<R>(x.x - y.y, x.y + y.x);
     else z[c] = mk<R>(x.x + y.y, y.x - x.y);
   }
+}
+template <int N, typename R>
+__device__ __forceinline__ void herm_pair(cx<R> (&z)[Sched<N>::E], const cx<R> *X, const cx<R> *Y, int t) {
+  HermRaw<N, R> h;
+  herm_load<N, R>(h, X, Y, t);
+  herm_combine<N, R>(z, h, t);
 }
 
 // Split the forward FFT F of (a + i b), a and b real, into the half spectra
@@ -286,11 +311,12 @@ __device__ __forceinline__ void split_store(const cx<R> (&f)[Sched<N>::E], cx<R>
 // One row of the fused z pass: c2r of u and omega (MODE 0: omega_z given in
 // oz0, u from the stale physical velocity), u x omega, forward transform of
 // (N_x, N_y) stored; N_z returned in nz.  Loads one Hermitian pair at a
-// time (register budget).
+// time (register budget).  MODE 1, 2: h holds the raw loads of the row's
+// first pair (W fields 0, 1) on entry.
 template <int N, typename R, int MODE, typename UR>
 __device__ __forceinline__ void zrow(cx<R> *__restrict__ w, UR *__restrict__ usr, Geo g, R invn3,
                                      const cx<R> *__restrict__ tw, int t, cx<R> *lds, PencilSlot<N> slot,
-                                     const R (&oz0)[Sched<N>::E], R (&nz)[Sched<N>::E]) {
+                                     const R (&oz0)[Sched<N>::E], R (&nz)[Sched<N>::E], const HermRaw<N, R> &h) {
   constexpr int E = Sched<N>::E, TPP = N / E;
   R u[E], v[E], ww[E], ox[E], oy[E], oz[E];
   if constexpr (MODE == 0) {
@@ -312,7 +338,7 @@ __device__ __forceinline__ void zrow(cx<R> *__restrict__ w, UR *__restrict__ usr
     }
   } else {
     cx<R> z1[E], z2[E], z3[E];
-    herm_pair<N, R>(z1, w, w + g.fs, t);
+    herm_combine<N, R>(z1, h, t);
     fft_pencil<N, R, true>(z1, t, lds, slot, tw);
     herm_pair<N, R>(z2, w + 2 * g.fs, w + 3 * g.fs, t);
     fft_pencil<N, R, true>(z2, t, lds, slot, tw);
@@ -356,8 +382,7 @@ __device__ __forceinline__ void zrow(cx<R> *__restrict__ w, UR *__restrict__ usr
 // kz < kmax only (the rest is removed by the dealias mask).
 // (N_x, N_y) are transformed and stored before N_z is transformed, so the
 // two forward transforms are never live together (128 instead of 137
-// VGPRs in float: four waves per SIMD instead of three).
-// Grid n*n/PPW blocks.
+// VGPRs in float).  Grid n*n/PPW blocks.
 template <int N, typename R, int PPW, int MODE, typename UR, int MINW = 1>
 __global__ __launch_bounds__(PPW *(N / Sched<N>::E), MINW) void k_zfused(cx<R> *__restrict__ W, UR *__restrict__ us,
                                                                     Geo g, R invn3,
@@ -370,14 +395,17 @@ __global__ __launch_bounds__(PPW *(N / Sched<N>::E), MINW) void k_zfused(cx<R> *
   cx<R> *w = W + blockIdx.z * g.mW + row * g.nfp;
   const PencilSlot<N> slot{p * padlen<N>()};
   R oz0[E], nz[E];
+  HermRaw<N, R> h;
   if constexpr (MODE == 0) {
     cx<R> z[E];
     herm_pair<N, R>(z, w + 5 * g.fs, nullptr, t);
     fft_pencil<N, R, true>(z, t, lds, slot, tw);
 #pragma unroll
     for (int c = 0; c < E; c++) oz0[c] = z[c].x * invn3;
+  } else {
+    herm_load<N, R>(h, w, w + g.fs, t);
   }
-  zrow<N, R, MODE, UR>(w, us + row * N, g, invn3, tw, t, lds, slot, oz0, nz);
+  zrow<N, R, MODE, UR>(w, us + row * N, g, invn3, tw, t, lds, slot, oz0, nz, h);
   // N_z is the transform of a real pencil: its first half is the r2c output.
   cx<R> f2[E];
 #pragma unroll
@@ -448,11 +476,11 @@ __global__ __launch_bounds__(PPW *(N / Sched<N>::E)) void k_zc2r(const cx<R> *__
 //  STAGE 0  : Ur = Un;  Ur' = Un + b dU;  acc = Un + a dU
 //  STAGE 1,2: Ur' = Un + b dU;  acc += a dU
 //  STAGE 3  : Un = acc + a dU                     (main.c:400-420)
-// Mixed precision (SR = float) keeps the accumulation in double but stores
-// its terms instead of the running sum: `acc` then holds, per member, a
-// float copy of Un and the float increments dU_0..dU_2 (each already only
-// float-accurate, being made from the float nonlinear term), and stage 3
-// forms Un + a0 dU0 + a1 dU1 + a2 dU2 + a3 dU3 in double in main.c's order.
+// Mixed precision (SR = float) keeps Un and Un+1 in double but accumulates
+// the increments in a float running sum: `acc` then holds, per member, a
+// float copy of Un and S = a0 dU0 + a1 dU1 + a2 dU2 (each dU is only float-
+// accurate anyway, being made from the float nonlinear term); stage 3 forms
+// (Un + S) + a3 dU3 in double.
 // The epilogue runs in double whatever R is.  phat (optional) receives P.
 // The new state (Ur' for stages 0..2, Un for stage 3) is returned in
 // U,V,Wv as the next stage's pass A would read it back from memory.
@@ -473,16 +501,16 @@ __device__ __forceinline__ void forward_x_rk(const cx<R> *__restrict__ W, cx<SR>
   const long soff = (long)j * N * g.nfp + k, sstr = g.nfp;
   const TileSlot<N, T> slot{p};
   constexpr bool MIXED_STATE = sizeof(SR) == 4;
-  // two positions per batch (six or nine loads; one position for the
-  // fifteen loads of the mixed stage 3 and for 16-element threads), next
-  // batch prefetched in float tiles of up to 8 elements (register budget)
-  constexpr int BC = EPI ? (EPI & 63) : ((MIXED_STATE && STAGE == 3) || E >= 16) ? 1 : 2;
+  // two positions per batch (six or nine loads; one position for
+  // 16-element threads), next batch prefetched in float tiles of up to 8
+  // elements (register budget)
+  constexpr int BC = EPI ? (EPI & 63) : E >= 16 ? 1 : 2;
   constexpr bool PF = EPI ? (EPI & 64) != 0 : sizeof(R) == 4 && E < 16;
   static_assert(E % BC == 0, "batch must divide the elements per thread");
   W += blockIdx.z * g.mW;
   uhr += blockIdx.z * g.mS;
   uhn += blockIdx.z * g.mS;
-  float2 *inc = (float2 *)acc + blockIdx.z * 4 * g.mS;  // mixed: [Un | dU0 | dU1 | dU2]
+  float2 *inc = (float2 *)acc + blockIdx.z * 2 * g.mS;  // mixed: [Un | S]
   acc += blockIdx.z * g.mS;
   if (phat) phat += blockIdx.z * g.fs;
   const double2 nd = q.nudt[blockIdx.z];
@@ -522,7 +550,7 @@ __device__ __forceinline__ void forward_x_rk(const cx<R> *__restrict__ W, cx<SR>
     double2 un[3][BC];  // FP64 stages 1,2 and mixed stage 3: Un
     float2 unf[3][BC];  // mixed stages 1,2: float copy of Un
     double2 ac[3][BC];  // FP64 stages 1..3: accumulator
-    float2 dd[3][3][BC];  // mixed stage 3: dU0..dU2
+    float2 sf[3][BC];   // mixed stages 1..3: running sum S
   };
   constexpr int NB = E / BC;
   auto load = [&](Batch &B, auto BB) {
@@ -537,11 +565,8 @@ __device__ __forceinline__ void forward_x_rk(const cx<R> *__restrict__ W, cx<SR>
         else B.ur[f][i] = uhr[lf];
         if constexpr (MIXED_STATE) {
           if constexpr (STAGE == 1 || STAGE == 2) B.unf[f][i] = inc[lf];
-          if constexpr (STAGE == 3) {
-            B.un[f][i] = uhn[lf];
-#pragma unroll
-            for (int s = 0; s < 3; s++) B.dd[s][f][i] = inc[(3 + 3 * s) * g.fs + lf];
-          }
+          if constexpr (STAGE == 3) B.un[f][i] = uhn[lf];
+          if constexpr (STAGE > 0) B.sf[f][i] = inc[3 * g.fs + lf];
         } else {
           if constexpr (STAGE == 1 || STAGE == 2) B.un[f][i] = uhn[lf];
           if constexpr (STAGE > 0) B.ac[f][i] = acc[lf];
@@ -584,23 +609,19 @@ __device__ __forceinline__ void forward_x_rk(const cx<R> *__restrict__ W, cx<SR>
             const cx<SR> nr = cvt<SR, double>(make_double2(ur.x + q.b * du.x, ur.y + q.b * du.y));
             st<SR>(uhr + lf, nr);
             st<float>(inc + lf, cvt<float, double>(ur));
-            st<float>(inc + 3 * g.fs + lf, cvt<float, double>(du));
+            st<float>(inc + 3 * g.fs + lf, cvt<float, double>(make_double2(q.a * du.x, q.a * du.y)));
             nxt[f] = cvt<R, SR>(nr);
           } else if constexpr (STAGE < 3) {
             const double2 un = cvt<double, float>(B.unf[f][i]);
+            const double2 sa = cvt<double, float>(B.sf[f][i]);
             const cx<SR> nr = cvt<SR, double>(make_double2(un.x + q.b * du.x, un.y + q.b * du.y));
             st<SR>(uhr + lf, nr);
-            st<float>(inc + (3 + 3 * STAGE) * g.fs + lf, cvt<float, double>(du));
+            st<float>(inc + 3 * g.fs + lf, cvt<float, double>(make_double2(sa.x + q.a * du.x, sa.y + q.a * du.y)));
             nxt[f] = cvt<R, SR>(nr);
           } else {
-            const double a0 = 1 / 6.0, a1 = 1 / 3.0, a2 = 1 / 3.0;  // main.c:15
             const double2 un = B.un[f][i];
-            const double2 d0 = cvt<double, float>(B.dd[0][f][i]);
-            const double2 d1 = cvt<double, float>(B.dd[1][f][i]);
-            const double2 d2 = cvt<double, float>(B.dd[2][f][i]);
-            double2 nn = make_double2(un.x + a0 * d0.x, un.y + a0 * d0.y);
-            nn = make_double2(nn.x + a1 * d1.x, nn.y + a1 * d1.y);
-            nn = make_double2(nn.x + a2 * d2.x, nn.y + a2 * d2.y);
+            const double2 sa = cvt<double, float>(B.sf[f][i]);
+            double2 nn = make_double2(un.x + sa.x, un.y + sa.y);
             nn = make_double2(nn.x + q.a * du.x, nn.y + q.a * du.y);
             st<double>(uhn + lf, nn);
             nxt[f] = cvt<R, double>(nn);

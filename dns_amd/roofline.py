@@ -66,14 +66,14 @@ def slot_bytes(n, precision="fp64"):
                 e = 3 * yc * cr + 3 * uc * csr + 6 * uc * c64 + 3 * uc * csr + 3 * uc * c64
             else:
                 e = 3 * yc * cr + 3 * uc * csr + 3 * uc * c64 + 3 * uc * c64
-        else:  # float copy of Un and float increments dU0..dU2; Un, Un+1 double
+        else:  # float copy of Un and float running sum S; Un, Un+1 double
             c32 = 8
-            if st == 0:
+            if st == 0:    # r Un(f64); w Ur(SR), Un copy, S
                 e = 3 * yc * cr + 3 * uc * c64 + 3 * uc * csr + 6 * uc * c32
-            elif st < 3:
-                e = 3 * yc * cr + 3 * uc * csr + 3 * uc * c32 + 3 * uc * csr + 3 * uc * c32
-            else:
-                e = 3 * yc * cr + 3 * uc * csr + 3 * uc * c64 + 9 * uc * c32 + 3 * uc * c64
+            elif st < 3:   # r Ur, Un copy, S; w Ur, S
+                e = 3 * yc * cr + 3 * uc * csr + 6 * uc * c32 + 3 * uc * csr + 3 * uc * c32
+            else:          # r Ur, Un(f64), S; w Un+1(f64)
+                e = 3 * yc * cr + 3 * uc * csr + 3 * uc * c64 + 3 * uc * c32 + 3 * uc * c64
         # E fused with the next stage's A: A's writes without its read
         nxt_writes = 3 * uc * cr if st == 3 else 6 * uc * cr
         for p, v in zip(PASSES, (a, b, c, d, e, e + nxt_writes)):
