@@ -213,6 +213,7 @@ static int ctx_autotune(dns_ctx *c, int steps, int *best_out, double *best_ms) {
     }
     hipMemcpyAsync(c->uhn, su, sb, hipMemcpyDeviceToDevice, c->stream);
     hipMemcpyAsync(c->us, sus, ub, hipMemcpyDeviceToDevice, c->stream);
+    c->impl->state_changed(c, -1);
     hipStreamSynchronize(c->stream);
     hipEventDestroy(a);
     hipEventDestroy(b);

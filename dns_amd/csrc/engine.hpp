@@ -108,6 +108,9 @@ struct Impl {
   virtual int irfft3(dns_ctx *c, const double *in, double *out) = 0;
   virtual int get_us(dns_ctx *c, int m, double *out) = 0;
   virtual int set_us(dns_ctx *c, int m, const double *in) = 0;
+  // After Un of member m (m < 0: all members) was set from outside: refresh
+  // the float copy of Un that mixed precision keeps.
+  virtual int state_changed(dns_ctx *c, int m) = 0;
   // spectral state of member m <-> host [3][n][n][n/2+1] complex (x-major)
   virtual int get_spectral(dns_ctx *c, int m, double *out) = 0;
   virtual int set_spectral(dns_ctx *c, int m, const double *in) = 0;
@@ -236,6 +239,19 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
     return DNS_OK;
   }
 
+  int state_changed(dns_ctx *c, int m) override {
+    if constexpr (sizeof(SR) == 4) {
+      const long m0 = m < 0 ? 0 : m, m1 = m < 0 ? c->members : m + 1;
+      for (long k = m0; k < m1; k++) {
+        const double *un = (const double *)((double2 *)c->uhn + k * c->g.mS);
+        float *cp = (float *)((float2 *)c->acc + k * 2 * c->g.mS);
+        k_convert<double, float><<<4096, 256, 0, c->stream>>>(un, cp, 2 * c->g.mS);
+        DNS_LAUNCHED();
+      }
+    }
+    return DNS_OK;
+  }
+
   int load(dns_ctx *c, int m, const double *uvw) override {
     c->w_ready = 0;
     hipStream_t s = c->stream;
@@ -260,6 +276,8 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
     k_xplain<N, double, T_D, false, double, false, true><<<grid_x(T_D), T_D * TPP, 0, s>>>(uh, ws, 3, c->g, twD(c));
     DNS_LAUNCHED();
     DNS_HIP(hipMemcpyAsync(uh, ws, 3 * sizeof(double2) * c->g.fs, hipMemcpyDeviceToDevice, s));
+    int rc = state_changed(c, m);
+    if (rc) return rc;
     DNS_HIP(hipStreamSynchronize(s));
     return DNS_OK;
   }
@@ -286,6 +304,8 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
                              hipMemcpyHostToDevice, s));
     k_swap_rows<double2><<<N * N, 256, 0, s>>>(ws, (double2 *)c->uhn + m * c->g.mS, N, c->g, 3);
     DNS_LAUNCHED();
+    int rc = state_changed(c, m);
+    if (rc) return rc;
     DNS_HIP(hipStreamSynchronize(s));
     return DNS_OK;
   }

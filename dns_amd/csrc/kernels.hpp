@@ -480,7 +480,8 @@ __global__ __launch_bounds__(PPW *(N / Sched<N>::E)) void k_zc2r(const cx<R> *__
 // the increments in a float running sum: `acc` then holds, per member, a
 // float copy of Un and S = a0 dU0 + a1 dU1 + a2 dU2 (each dU is only float-
 // accurate anyway, being made from the float nonlinear term); stage 3 forms
-// (Un + S) + a3 dU3 in double.
+// (Un + S) + a3 dU3 in double and also writes its float copy, which stages
+// 0..2 of the next step read (whoever sets Un from outside refreshes it).
 // The epilogue runs in double whatever R is.  phat (optional) receives P.
 // The new state (Ur' for stages 0..2, Un for stage 3) is returned in
 // U,V,Wv as the next stage's pass A would read it back from memory.
@@ -543,8 +544,9 @@ __device__ __forceinline__ void forward_x_rk(const cx<R> *__restrict__ W, cx<SR>
   // stores: the stores may alias the loads as far as the compiler can tell,
   // so a per-element order would cost one memory round trip per element
   // (and each wait would also drain the earlier stores).
-  using URT = std::conditional_t<STAGE == 0, double2, cx<SR>>;  // Ur as stored
-  using URS = std::conditional_t<STAGE == 0, double, SR>;
+  // Ur as stored: stage 0 reads Un (double; mixed: its float copy)
+  using URT = std::conditional_t<STAGE == 0 && !MIXED_STATE, double2, cx<SR>>;
+  using URS = std::conditional_t<STAGE == 0 && !MIXED_STATE, double, SR>;
   struct Batch {
     URT ur[3][BC];      // stage state (stage 0: Un)
     double2 un[3][BC];  // FP64 stages 1,2 and mixed stage 3: Un
@@ -561,7 +563,8 @@ __device__ __forceinline__ void forward_x_rk(const cx<R> *__restrict__ W, cx<SR>
 #pragma unroll
       for (int f = 0; f < 3; f++) {
         const long lf = f * g.fs + soff + (t + TPP * (c0 + i)) * sstr;
-        if constexpr (STAGE == 0) B.ur[f][i] = uhn[lf];
+        if constexpr (STAGE == 0 && MIXED_STATE) B.ur[f][i] = inc[lf];
+        else if constexpr (STAGE == 0) B.ur[f][i] = uhn[lf];
         else B.ur[f][i] = uhr[lf];
         if constexpr (MIXED_STATE) {
           if constexpr (STAGE == 1 || STAGE == 2) B.unf[f][i] = inc[lf];
@@ -608,7 +611,6 @@ __device__ __forceinline__ void forward_x_rk(const cx<R> *__restrict__ W, cx<SR>
           if constexpr (STAGE == 0) {
             const cx<SR> nr = cvt<SR, double>(make_double2(ur.x + q.b * du.x, ur.y + q.b * du.y));
             st<SR>(uhr + lf, nr);
-            st<float>(inc + lf, cvt<float, double>(ur));
             st<float>(inc + 3 * g.fs + lf, cvt<float, double>(make_double2(q.a * du.x, q.a * du.y)));
             nxt[f] = cvt<R, SR>(nr);
           } else if constexpr (STAGE < 3) {
@@ -624,6 +626,7 @@ __device__ __forceinline__ void forward_x_rk(const cx<R> *__restrict__ W, cx<SR>
             double2 nn = make_double2(un.x + sa.x, un.y + sa.y);
             nn = make_double2(nn.x + q.a * du.x, nn.y + q.a * du.y);
             st<double>(uhn + lf, nn);
+            st<float>(inc + lf, cvt<float, double>(nn));
             nxt[f] = cvt<R, double>(nn);
           }
         } else if constexpr (STAGE == 0) {

@@ -12,6 +12,7 @@ spectral field and Rr = n^3 reals per physical field:
   D     r 3Zc w 3Yc             as 1                      as 1
   E     r 3Yc+3Uc(f64)          r 3Yc+3Uc(SR)+6Uc(f64)    r 3Yc+3Uc(SR)+3Uc(f64)
         w 3Uc(SR)+3Uc(f64)      w 3Uc(SR)+3Uc(f64)        w 3Uc(f64)
+        (FP64; the mixed-precision state terms are listed in slot_bytes)
 
 where Zc = n^2 nk and Yc = n (2nk-1) nk count the dealiased part of the
 nonlinear term (nk = #{m >= 0 : m < kmax}, kmax = 2/3 (n/2+1)).
@@ -68,12 +69,12 @@ def slot_bytes(n, precision="fp64"):
                 e = 3 * yc * cr + 3 * uc * csr + 3 * uc * c64 + 3 * uc * c64
         else:  # float copy of Un and float running sum S; Un, Un+1 double
             c32 = 8
-            if st == 0:    # r Un(f64); w Ur(SR), Un copy, S
-                e = 3 * yc * cr + 3 * uc * c64 + 3 * uc * csr + 6 * uc * c32
+            if st == 0:    # r Un copy; w Ur(SR), S
+                e = 3 * yc * cr + 3 * uc * c32 + 3 * uc * csr + 3 * uc * c32
             elif st < 3:   # r Ur, Un copy, S; w Ur, S
                 e = 3 * yc * cr + 3 * uc * csr + 6 * uc * c32 + 3 * uc * csr + 3 * uc * c32
-            else:          # r Ur, Un(f64), S; w Un+1(f64)
-                e = 3 * yc * cr + 3 * uc * csr + 3 * uc * c64 + 3 * uc * c32 + 3 * uc * c64
+            else:          # r Ur, Un(f64), S; w Un+1(f64) and its float copy
+                e = 3 * yc * cr + 3 * uc * csr + 3 * uc * c64 + 3 * uc * c32 + 3 * uc * c64 + 3 * uc * c32
         # E fused with the next stage's A: A's writes without its read
         nxt_writes = 3 * uc * cr if st == 3 else 6 * uc * cr
         for p, v in zip(PASSES, (a, b, c, d, e, e + nxt_writes)):
