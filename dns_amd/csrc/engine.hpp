@@ -190,16 +190,23 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
   // elements per thread where the schedule allows (powers of two from 256
   // on); float y passes use up to 1024 threads.
   static constexpr int T_D = 8 * TPP > 512 ? 512 / TPP : 8;
-  // inverse y pass: 8 double / 16 float columns (128-byte row segments;
+  // y passes: double grids whose pencils need more than 64 threads (n=1024)
+  // take 16 elements per thread, keeping 8-column tiles in 512 threads
+  // (n=1024: inverse 25.0 -> 22.5 ms, forward 7.7 -> 5.7 ms).
+  static constexpr int E_Y = (sizeof(R) == 8 && TPP > 64) ? 2 * E : E;
+  static constexpr int TPP_Y = N / E_Y;
+  // 8 double / 16 float columns (float y passes use up to 1024 threads;
   // narrower float tiles over-fetch: 64-byte segments read 1.8x their bytes)
-  static constexpr int T_Y = sizeof(R) == 8 ? T_D : (16 * TPP > 1024 ? 1024 / TPP : 16);
-  // forward (truncated) y pass in float: 16 columns, up to 1024 threads
-  // (its register needs are small enough)
-  static constexpr int T_YF = sizeof(R) == 8 ? T_D : (16 * TPP > 1024 ? 1024 / TPP : 16);
-  static constexpr int PIPE_YF = sizeof(R) == 8 ? 1 : 0;
-  // inverse y pass: fields loaded ahead (two in float: 1.47 -> 1.38 ms at
-  // n=512; one in double, where a second costs occupancy)
-  static constexpr int PIPE_Y = sizeof(R) == 8 ? 1 : 2;
+  static constexpr int T_Y = sizeof(R) == 8 ? (8 * TPP_Y > 512 ? 512 / TPP_Y : 8)
+                                            : (16 * TPP_Y > 1024 ? 1024 / TPP_Y : 16);
+  static constexpr int T_YF = T_Y;
+  // fields loaded ahead: inverse y pass two in float (1.47 -> 1.38 ms at
+  // n=512), one in double (a second costs occupancy), none with 16-element
+  // threads (register budget); the forward y pass one in double only
+  static constexpr int PIPE_Y = sizeof(R) == 8 ? (E_Y > E ? 0 : 1) : 2;
+  static constexpr int PIPE_YF = sizeof(R) == 8 ? (E_Y > E ? 0 : 1) : 0;
+  // plain transforms of the working precision (rfft3 / irfft3)
+  static constexpr int T_P = sizeof(R) == 8 ? T_D : (16 * TPP > 1024 ? 1024 / TPP : 16);
   static constexpr int E_X = (sizeof(R) == 4 && E == 8 && N >= 256) ? 16 : E;
   static constexpr int TPP_X = N / E_X;
   static constexpr int T_X = sizeof(R) == 8 ? T_D : (16 * TPP_X > 512 ? 512 / TPP_X : 16);
@@ -393,10 +400,10 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
     const R invn3 = (R)c->invn3;
     static const double a_rk[4] = {1 / 6.0, 1 / 3.0, 1 / 3.0, 1 / 6.0};  // main.c:15
     static const double b_rk[4] = {0.5, 0.5, 1.0, 0.0};                 // main.c:16
-    const dim3 gy = grid_x(TY, M), by(TY * TPP);
+    const dim3 gy = grid_x(TY, M), by(TY * TPP_Y);
     const dim3 gx = grid_x(TX, M), bx(TX * TPP_X);
     const dim3 gz(N * N / PP, 1, M), bz(PP * TPP);
-    const dim3 gy_trunc((g.nk + TYF - 1) / TYF, N, M), byf(TYF * TPP);  // dealiased kz columns only
+    const dim3 gy_trunc((g.nk + TYF - 1) / TYF, N, M), byf(TYF * TPP_Y);  // dealiased kz columns only
     for (long st = 0; st < nsteps; st++) {
       double2 *ph = (c->capture && st == nsteps - 1) ? (double2 *)c->phat : nullptr;
       for (int rk = 0; rk < 4; rk++) {
@@ -404,17 +411,17 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
         const int sl = DNS_NPASS * rk;
         // pass A of this stage ran fused into the previous stage's E
         if (rk == 0) {
-          DNS_TIMED(c, sl + 1, (k_ypass<N, R, TY, true, 1, PIPE_Y><<<gy, by, 0, s>>>(W + 3 * g.fs, 3, g, tw, g.mW)));
+          DNS_TIMED(c, sl + 1, (k_ypass<N, R, TY, true, 1, PIPE_Y, false, E_Y><<<gy, by, 0, s>>>(W + 3 * g.fs, 3, g, tw, g.mW)));
           DNS_TIMED(c, sl + 2, (k_zfused<N, R, PP, 0, R><<<gz, bz, 0, s>>>(W, (R *)c->us, g, invn3, tw)));
         } else {
-          DNS_TIMED(c, sl + 1, (k_ypass<N, R, TY, true, 1, PIPE_Y><<<gy, by, 0, s>>>(W, 6, g, tw, g.mW)));
+          DNS_TIMED(c, sl + 1, (k_ypass<N, R, TY, true, 1, PIPE_Y, false, E_Y><<<gy, by, 0, s>>>(W, 6, g, tw, g.mW)));
           if (rk < 3)
             DNS_TIMED(c, sl + 2, (k_zfused<N, R, PP, 1, R><<<gz, bz, 0, s>>>(W, (R *)c->us, g, invn3, tw)));
           else
             DNS_TIMED(c, sl + 2, (k_zfused<N, R, PP, 2, R><<<gz, bz, 0, s>>>(W, (R *)c->us, g, invn3, tw)));
         }
         DNS_TIMED(c, sl + 3,
-                  (k_ypass<N, R, TYF, false, 1, PIPE_YF, true><<<gy_trunc, byf, 0, s>>>(W, 3, g, tw, g.mW)));
+                  (k_ypass<N, R, TYF, false, 1, PIPE_YF, true, E_Y><<<gy_trunc, byf, 0, s>>>(W, 3, g, tw, g.mW)));
         // E fused with the next stage's A (the next step's stage 1 after rk 3)
         switch (rk) {
           case 0:
@@ -483,9 +490,9 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
     DNS_HIP(hipMemcpyAsync(phys, in, sizeof(double) * g.rs, hipMemcpyHostToDevice, s));
     k_zr2c<N, R, PPW><<<N * N / PPW, PPW * TPP, 0, s>>>(phys, spec, 1, g, twR(c));
     DNS_LAUNCHED();
-    k_ypass<N, R, T_Y, false><<<grid_x(T_Y), T_Y * TPP, 0, s>>>(spec, 1, g, twR(c));
+    k_ypass<N, R, T_P, false><<<grid_x(T_P), T_P * TPP, 0, s>>>(spec, 1, g, twR(c));
     DNS_LAUNCHED();
-    k_xplain<N, R, T_Y, false, R><<<grid_x(T_Y), T_Y * TPP, 0, s>>>(spec, spec, 1, g, twR(c));
+    k_xplain<N, R, T_P, false, R><<<grid_x(T_P), T_P * TPP, 0, s>>>(spec, spec, 1, g, twR(c));
     DNS_LAUNCHED();
     double2 *specd = (double2 *)spec;
     if (sizeof(R) != 8) {  // widen into the region after the float spectrum
@@ -510,9 +517,9 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
     DNS_HIP(hipMemset2DAsync(specd, sizeof(double2) * g.nfp, 0, sizeof(double2) * g.nfp, (size_t)N * N, s));
     DNS_HIP(hipMemcpy2DAsync(specd, sizeof(double2) * g.nfp, in, sizeof(double2) * NF, sizeof(double2) * NF,
                              (size_t)N * N, hipMemcpyHostToDevice, s));
-    k_xplain<N, R, T_Y, true, double><<<grid_x(T_Y), T_Y * TPP, 0, s>>>(specd, tmpc, 1, g, twR(c));
+    k_xplain<N, R, T_P, true, double><<<grid_x(T_P), T_P * TPP, 0, s>>>(specd, tmpc, 1, g, twR(c));
     DNS_LAUNCHED();
-    k_ypass<N, R, T_Y, true><<<grid_x(T_Y), T_Y * TPP, 0, s>>>(tmpc, 1, g, twR(c));
+    k_ypass<N, R, T_P, true><<<grid_x(T_P), T_P * TPP, 0, s>>>(tmpc, 1, g, twR(c));
     DNS_LAUNCHED();
     k_zc2r<N, R, PPW><<<N * N / PPW, PPW * TPP, 0, s>>>(tmpc, phys, g, (R)1, twR(c));
     DNS_LAUNCHED();
