@@ -235,13 +235,74 @@ def validate(a):
     return 0
 
 
+def iso_field(n, seed=20261202):
+    """Seeded isotropic field of dns-ic (k0=4, kinetic energy 0.5)."""
+    import subprocess
+    import tempfile
+    exe = os.path.join(ROOT, "dns_amd", "bin", "dns-ic")
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "iso.raw")
+        subprocess.run([exe, "-t", "iso", "-N", str(n), "-S", str(seed), "-o", path], check=True,
+                       capture_output=True)
+        return np.fromfile(path, dtype="<f8").reshape(4, n, n, n)[:3].copy()
+
+
+def configs(a):
+    """The long runs of the measurement plan, FP64 and mixed side by side:
+    Taylor-Green and the seeded isotropic field at n=256 (nu=6.25e-4,
+    dt=1e-3, 2000 steps) and Taylor-Green at n=512 (nu=1/3000, dt=5e-4,
+    500 steps).  Reported per run: steps/s of each precision (graph
+    replay, diagnostics every 10 steps as the command line does), the
+    largest relative deviation of the mixed energy and enstrophy from the
+    FP64 run over all diagnostics lines, and the max-norm deviation of the
+    final physical velocity relative to max |u|."""
+    import dns_amd
+    runs = [("tgv", 256, 6.25e-4, 1e-3, 2000), ("iso", 256, 6.25e-4, 1e-3, 2000),
+            ("tgv", 512, 1 / 3000, 5e-4, 500)]
+    out = []
+    for ic, n, nu, dt, steps in runs:
+        u = tgv(n) if ic == "tgv" else iso_field(n)
+        res = {"ic": ic, "n": n, "nu": nu, "dt": dt, "steps": steps}
+        hist, finals = {}, {}
+        for prec in ("fp64", "mixed"):
+            print(f"configs: {ic} n={n} {prec}", file=sys.stderr, flush=True)
+            with dns_amd.Solver(n, nu, dt, prec) as s:
+                s.load_velocity(u)
+                s.step(2)
+                s.load_velocity(u)  # graph and launch shapes warm, state reset
+                lines = [s.diagnostics()]
+                t0 = time.perf_counter()
+                for _ in range(steps // 10):
+                    s.step(10)
+                    lines.append(s.diagnostics())
+                wall = time.perf_counter() - t0
+                hist[prec] = lines
+                finals[prec] = s.get_stale_velocity()
+                res[f"{prec}_steps_per_s"] = steps / wall
+        de = max(abs(m[0] - f[0]) / f[0] for f, m in zip(hist["fp64"], hist["mixed"]))
+        dz = max(abs(m[1] - f[1]) / f[1] for f, m in zip(hist["fp64"], hist["mixed"]))
+        ua, ub = finals["fp64"], finals["mixed"]
+        res.update({"mixed_max_rel_E": de, "mixed_max_rel_Z": dz,
+                    "mixed_max_norm_u": float(np.abs(ub - ua).max() / np.abs(ua).max()),
+                    "E_final_fp64": hist["fp64"][-1][0], "Z_final_fp64": hist["fp64"][-1][1],
+                    "speedup": res["mixed_steps_per_s"] / res["fp64_steps_per_s"]})
+        out.append(res)
+        print(f"configs: {json.dumps(res)}", file=sys.stderr, flush=True)
+    worst = max(r["mixed_max_rel_E"] for r in out)
+    line = {"metric": "long runs, FP64 and mixed: max relative energy deviation of mixed", "value": worst,
+            "unit": "relative", "higher_is_better": False, "n_gpus": 1, "dtype": "f64 / f32+f64",
+            "data": "synthetic", "runs": out}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=512)
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--ensemble", action="store_true", help="batched ensemble workload instead")
@@ -249,11 +310,13 @@ def main():
     ap.add_argument("--adj-steps", type=int, default=200)
     ap.add_argument("--adj-dirs", type=int, default=8)
     ap.add_argument("--validate", action="store_true", help="n=256 golden-trajectory validation instead")
+    ap.add_argument("--configs", action="store_true", help="long FP64/mixed runs at n=256 and 512 instead")
     ap.add_argument("--val-steps", type=int, default=4000)
     a = ap.parse_args()
     if a.gpus != 1:
-        print("bench.py: this build measures one GPU only; multi-GPU runs are out of scope "
-              "(see the sharding design in DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU: runs on several GPUs are not supported (ensemble members "
+              "would shard with no communication; one large grid would need x/y slabs and an "
+              "all-to-all per transform)", file=sys.stderr)
         return 2
     if a.ensemble:
         return ensemble(a)
@@ -261,6 +324,8 @@ def main():
         return adjoint(a)
     if a.validate:
         return validate(a)
+    if a.configs:
+        return configs(a)
     import dns_amd
     from dns_amd import roofline
 

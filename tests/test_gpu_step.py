@@ -118,3 +118,31 @@ def test_mixed_energy_error_tgv():
             worst = max(worst, abs(eb - ea) / ea)
         print("mixed max rel energy error", worst)
         assert worst < 1e-5
+
+
+def test_mixed_isotropic_field_max_norm(tmp_path):
+    """Mixed vs FP64 on the seeded isotropic field of dns-ic (k0=4, E=0.5):
+    energy and enstrophy within 1e-5 relative, and the physical velocity
+    (the stale velocity after the step) within 1e-5 of max|u| in max norm."""
+    import subprocess
+    from conftest import DNS_IC
+    n, nu, dt = 64, 1 / 800, 2e-3
+    raw = tmp_path / "iso64.raw"
+    subprocess.run([DNS_IC, "-t", "iso", "-N", str(n), "-o", str(raw)], check=True, capture_output=True)
+    u = np.fromfile(raw, dtype="<f8").reshape(4, n, n, n)[:3].copy()
+    print("mixed vs fp64 isotropic n=64, 100 steps", flush=True)
+    with dns_amd.Solver(n, nu, dt, "fp64") as a, dns_amd.Solver(n, nu, dt, "mixed") as b:
+        a.load_velocity(u)
+        b.load_velocity(u)
+        we = wz = 0.0
+        for _ in range(10):
+            a.step(10)
+            b.step(10)
+            ea, za = a.diagnostics()
+            eb, zb = b.diagnostics()
+            we = max(we, abs(eb - ea) / ea)
+            wz = max(wz, abs(zb - za) / za)
+        ua, ub = a.get_stale_velocity(), b.get_stale_velocity()
+        wu = np.abs(ub - ua).max() / np.abs(ua).max()
+        print(f"mixed max rel error: E {we:.2e} Z {wz:.2e} u (max norm) {wu:.2e}")
+        assert we < 1e-5 and wz < 1e-5 and wu < 1e-5
