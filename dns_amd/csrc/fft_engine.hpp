@@ -163,18 +163,11 @@ template <typename R, int RAD, bool INV> __device__ __forceinline__ void dft(cx<
 // separately from global memory operations, so a transform that follows a
 // batch of global stores does not stall until those stores complete (a
 // global twiddle load would: loads and stores share one counter).
-#ifndef DNS_LDS_TWIDDLES
-#define DNS_LDS_TWIDDLES 1
-#endif
 template <int N, typename R>
 __device__ __forceinline__ const cx<R> *stage_twiddles(cx<R> *twl, const cx<R> *__restrict__ tw) {
-  if constexpr (DNS_LDS_TWIDDLES) {
-    for (int i = threadIdx.x; i < tw_size<N>(); i += blockDim.x) twl[i] = tw[i];
-    __syncthreads();
-    return twl;
-  } else {
-    return tw;
-  }
+  for (int i = threadIdx.x; i < tw_size<N>(); i += blockDim.x) twl[i] = tw[i];
+  __syncthreads();
+  return twl;
 }
 
 // ---------------------------------------------------------------------------

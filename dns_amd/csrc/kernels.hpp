@@ -22,9 +22,9 @@
 //   C  k_zfused    : c2r along z of u and omega, 1/n^3 scaling (348-352,
 //                    366-371), u x omega (372-377), r2c along z of N
 //   D  k_ypass     : forward FFT along y
-//   E  k_xfwd_rk   : forward FFT along x + dealias*dt (381-386), projection
-//                    and viscous term (387-399), RK update (400-413);
-//                    k_xfwd_rk_xinv fuses it with the next stage's A
+//   E  forward FFT along x + dealias*dt (381-386), projection and viscous
+//      term (387-399), RK update (400-413): k_xfwd_rk_xinv runs it fused
+//      with the next stage's A
 #pragma once
 #include "fft_engine.hpp"
 #include <type_traits>
@@ -670,23 +670,6 @@ __device__ __forceinline__ void forward_x_rk(const cx<R> *__restrict__ W, cx<SR>
       apply(cur, BB);
     });
   }
-}
-
-template <int N, typename R, int T, int STAGE, typename SR, int MINW = 1, int EPI = 0, int E = Sched<N>::E>
-__global__ __launch_bounds__(T *(N / E), MINW) void k_xfwd_rk(const cx<R> *__restrict__ W,
-                                                                   cx<SR> *__restrict__ uhr,
-                                                                   double2 *__restrict__ uhn,
-                                                                   double2 *__restrict__ acc,
-                                                                   double2 *__restrict__ phat, Geo g, RKParams q,
-                                                                   const cx<R> *__restrict__ tw) {
-  __shared__ cx<R> lds[T * padlen<N>()];
-  __shared__ cx<R> twl[tw_size<N>()];
-  tw = stage_twiddles<N, R>(twl, tw);
-  const int p = threadIdx.x % T, t = threadIdx.x / T;
-  const int j = blockIdx.y, k = blockIdx.x * T + p;
-  const bool ok = k < N / 2 + 1;
-  cx<R> U[E], V[E], Wv[E];
-  forward_x_rk<N, R, T, STAGE, SR, EPI, E>(W, uhr, uhn, acc, phat, g, q, tw, t, p, j, k, ok, lds, U, V, Wv);
 }
 
 // E of stage STAGE fused with A of the following stage (stage STAGE+1, or
