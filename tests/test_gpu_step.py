@@ -146,3 +146,25 @@ def test_mixed_isotropic_field_max_norm(tmp_path):
         wu = np.abs(ub - ua).max() / np.abs(ua).max()
         print(f"mixed max rel error: E {we:.2e} Z {wz:.2e} u (max norm) {wu:.2e}")
         assert we < 1e-5 and wz < 1e-5 and wu < 1e-5
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_state_transfer_continues_bitwise(precision):
+    """A state taken out with get_spectral / get_stale_velocity and put into
+    a second solver with set_spectral / set_stale_velocity continues bit for
+    bit (the library's internal state layout and, in mixed precision, its
+    float copy of the spectral state are rebuilt by the setters)."""
+    n, nu, dt = 48, 0.01, 0.004
+    u = random_field(n, 41)
+    with dns_amd.Solver(n, nu, dt, precision) as a, dns_amd.Solver(n, nu, dt, precision) as b:
+        a.load_velocity(u)
+        a.step(5)
+        b.load_velocity(random_field(n, 42))
+        b.step(2)
+        b.set_spectral(a.get_spectral())
+        b.set_stale_velocity(a.get_stale_velocity())
+        assert np.array_equal(b.get_spectral(), a.get_spectral())
+        a.step(6)
+        b.step(6)
+        assert np.array_equal(b.get_spectral(), a.get_spectral())
+        assert np.array_equal(b.get_stale_velocity(), a.get_stale_velocity())
