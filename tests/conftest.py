@@ -189,3 +189,33 @@ def parse_lines(text):
 def rel(a, b):
     a, b = np.asarray(a), np.asarray(b)
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def rk4_amplification(z):
+    """RK4 amplification factor of u' = -u over one step of size z."""
+    return 1 - z + z ** 2 / 2 - z ** 3 / 6 + z ** 4 / 24
+
+
+def beltrami_field(n, k=1, a=1.0, b=0.7, c=0.4):
+    """ABC (Beltrami) flow with wavenumber k: curl u = k u, so u x omega = 0
+    and the flow decays by viscosity alone."""
+    x = 2 * np.pi * np.arange(n) / n
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    return np.stack([a * np.sin(k * Z) + c * np.cos(k * Y),
+                     b * np.sin(k * X) + a * np.cos(k * Z),
+                     c * np.sin(k * Y) + b * np.cos(k * X)])
+
+
+def wavenumbers(n):
+    """kx, ky, kz of the half-spectrum layout [n][n][n/2+1] (Nyquist of x
+    and y at -n/2, main.c:224-230)."""
+    w = np.arange(n)
+    w = np.where(w < n // 2, w, w - n).astype(float)
+    return w[:, None, None], w[None, :, None], np.arange(n // 2 + 1, dtype=float)[None, None, :]
+
+
+def divergence_hat(uh):
+    """k . U over the stored half spectrum (without the factor i)."""
+    n = uh.shape[1]
+    kx, ky, kz = wavenumbers(n)
+    return kx * uh[0] + ky * uh[1] + kz * uh[2]

@@ -7,7 +7,8 @@ import numpy as np
 import pytest
 
 import dns_amd
-from conftest import load_golden, random_field, rel, tgv_field
+from conftest import (beltrami_field, divergence_hat, load_golden, random_field, rel, rk4_amplification,
+                      tgv_field, wavenumbers)
 
 pytestmark = pytest.mark.gpu
 
@@ -168,3 +169,34 @@ def test_state_transfer_continues_bitwise(precision):
         b.step(6)
         assert np.array_equal(b.get_spectral(), a.get_spectral())
         assert np.array_equal(b.get_stale_velocity(), a.get_stale_velocity())
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("mixed", 1e-6)])
+def test_beltrami_flow_decays_by_rk4_polynomial(precision, tol):
+    """curl u = k u, so u x omega = 0: the state decays by R(nu dt k^2)."""
+    n, nu, dt, k, m = 32, 0.02, 0.05, 3, 20
+    print(f"beltrami {precision} n={n}", flush=True)
+    with dns_amd.Solver(n, nu, dt, precision) as s:
+        s.load_velocity(beltrami_field(n, k))
+        u0 = s.get_spectral()
+        s.step(m)
+        want = u0 * rk4_amplification(nu * dt * k * k) ** m
+        assert np.max(np.abs(s.get_spectral() - want)) / np.max(np.abs(u0)) < tol
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("mixed", 1e-6)])
+def test_divergence_decays_by_rk4_polynomial(precision, tol):
+    """The projection removes the divergent part of N, so k.U of any field
+    evolves by R(nu dt |k|^2) per step."""
+    n, nu, dt, m = 32, 0.02, 0.01, 8
+    rng = np.random.default_rng(18)
+    u = rng.standard_normal((3, n, n, n))  # not solenoidal
+    print(f"divergence {precision} n={n}", flush=True)
+    with dns_amd.Solver(n, nu, dt, precision) as s:
+        s.load_velocity(u)
+        d0 = divergence_hat(s.get_spectral())
+        s.step(m)
+        kx, ky, kz = wavenumbers(n)
+        want = d0 * rk4_amplification(nu * dt * (kx ** 2 + ky ** 2 + kz ** 2)) ** m
+        got = divergence_hat(s.get_spectral())
+        assert np.max(np.abs(got - want)) / np.max(np.abs(d0)) < tol

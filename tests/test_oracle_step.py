@@ -4,7 +4,13 @@ excerpted in tests/golden/) and against closed forms:
   - Taylor-Green E(0) = 1/8, Z(0) = 3/8 (poc/tgv.mac of the reference);
   - the linear (Stokes) limit: a shear flow whose u x omega is a pure
     gradient decays mode by mode with the RK4 amplification polynomial
-    R(z) = 1 - z + z^2/2 - z^3/6 + z^4/24, z = nu dt |k|^2.
+    R(z) = 1 - z + z^2/2 - z^3/6 + z^4/24, z = nu dt |k|^2;
+  - a Beltrami (ABC) flow, curl u = k u, so u x omega = 0 at every stage:
+    the whole state decays by R(nu dt k^2) per step (pins the curl and the
+    cross product: a swapped component or index leaves a nonzero N);
+  - the divergence k.U of any field evolves mode by mode by R(nu dt |k|^2)
+    per step whatever the nonlinear term does, because the projection
+    removes the divergent part of N (pins the projection).
 The golden runs also pin the stale-velocity rule of stage 1 (main.c:344-353):
 a solver that refreshes u at stage 1 drifts by ~3e-13 within 10 steps at
 n=128 and fails the 1e-13 bound used here."""
@@ -15,7 +21,8 @@ import subprocess
 import numpy as np
 import pytest
 
-from conftest import ORACLE_CLI, load_golden, parse_lines, tgv_field, write_raw
+from conftest import (ORACLE_CLI, beltrami_field, divergence_hat, load_golden, parse_lines,
+                      rk4_amplification, tgv_field, wavenumbers, write_raw)
 
 
 def test_tgv_initial_diagnostics(oracle):
@@ -66,3 +73,27 @@ def test_stokes_limit_rk4_polynomial(oracle):
     # everything else stays at round-off
     uh[0, 0, [1, 2, n - 1, n - 2], 0] = 0
     assert np.max(np.abs(uh)) / n ** 3 < 1e-15
+
+
+def test_beltrami_flow_decays_by_rk4_polynomial(oracle):
+    n, nu, dt, k, m = 16, 0.02, 0.05, 2, 12
+    s = oracle.solver(n, nu, dt)
+    s.load_velocity(beltrami_field(n, k))
+    u0 = s.get_spectral()
+    s.step(m)
+    want = u0 * rk4_amplification(nu * dt * k * k) ** m
+    assert np.max(np.abs(s.get_spectral() - want)) / np.max(np.abs(u0)) < 1e-13
+
+
+def test_divergence_decays_by_rk4_polynomial(oracle):
+    n, nu, dt, m = 16, 0.02, 0.02, 6
+    rng = np.random.default_rng(17)
+    u = rng.standard_normal((3, n, n, n))  # not solenoidal
+    s = oracle.solver(n, nu, dt)
+    s.load_velocity(u)
+    d0 = divergence_hat(s.get_spectral())
+    s.step(m)
+    kx, ky, kz = wavenumbers(n)
+    want = d0 * rk4_amplification(nu * dt * (kx ** 2 + ky ** 2 + kz ** 2)) ** m
+    got = divergence_hat(s.get_spectral())
+    assert np.max(np.abs(got - want)) / np.max(np.abs(d0)) < 1e-13
