@@ -89,8 +89,13 @@ __global__ __launch_bounds__(256) void k_swap_rows(const T *__restrict__ src, T 
 // TRUNC (forward transform of the nonlinear term only): the columns
 // k >= g.nk are not read and only rows inside the dealiased box are written;
 // the grid then covers ceil(nk/T) column tiles.
+// CURLX (inverse transform of u, v, w and omega, nfields = 6): field 3 is not
+// read; curlX = I*(ky w - kz v) (main.c:361, same operation order) is formed
+// from fields 1 and 2 as loaded, then transformed and stored as field 3.
+// In (x, ky, kz) space ky and kz are per-element constants, so this equals
+// the inverse x transform of the spectral curlX.
 template <int N, typename R, int T, bool INV, int MINW = 1, int PIPE = 0, bool TRUNC = false,
-          int E = Sched<N>::E>
+          int E = Sched<N>::E, bool CURLX = false>
 __global__ __launch_bounds__(T *(N / E), MINW) void k_ypass(cx<R> *__restrict__ data, int nfields, Geo g,
                                                            const cx<R> *__restrict__ tw, long mstride = 0) {
   constexpr int TPP = N / E;
@@ -101,17 +106,15 @@ __global__ __launch_bounds__(T *(N / E), MINW) void k_ypass(cx<R> *__restrict__ 
   const int k = blockIdx.x * T + p;
   const bool ok = k < (TRUNC ? g.nk : N / 2 + 1);
   cx<R> *base = data + blockIdx.z * mstride + (long)blockIdx.y * N * g.nfp + k;
+  // loaded fields: all, or 0,1,2,4,5 with CURLX
+  const int nload = CURLX ? 5 : nfields;
+  auto fld = [](int i) { return CURLX && i >= 3 ? i + 1 : i; };
   auto load = [&](cx<R>(&v)[E], int f) {
     const cx<R> *d = base + f * g.fs;
 #pragma unroll
     for (int c = 0; c < E; c++) v[c] = ok ? d[(long)(t + TPP * c) * g.nfp] : zero<R>();
   };
-  cx<R> v[E], nx[E], nx2[E];
-  load(v, 0);
-  if (PIPE >= 2 && nfields > 1) load(nx, 1);
-  for (int f = 0; f < nfields; f++) {
-    if (PIPE == 1 && f + 1 < nfields) load(nx, f + 1);
-    if (PIPE == 2 && f + 2 < nfields) load(nx2, f + 2);
+  auto fft_store = [&](cx<R>(&v)[E], int f) {
     fft_pencil<N, R, INV>(v, t, lds, TileSlot<N, T>{p}, tw);
     if (ok) {
       cx<R> *d = base + f * g.fs;
@@ -119,7 +122,34 @@ __global__ __launch_bounds__(T *(N / E), MINW) void k_ypass(cx<R> *__restrict__ 
       for (int c = 0; c < E; c++)
         if (!TRUNC || in_box(t + TPP * c, N, g.nk)) st<R>(d + (long)(t + TPP * c) * g.nfp, v[c]);
     }
-    if (f + 1 < nfields) {
+  };
+  cx<R> v[E], nx[E], nx2[E];
+  cx<R> ox[E];  // CURLX: kz * v of field 1, then curlX
+  load(v, 0);
+  if (PIPE >= 2 && nload > 1) load(nx, fld(1));
+  // not unrolled: an unrolled loop lets loads of later fields move up and
+  // raises the register count
+#pragma unroll 1
+  for (int i = 0; i < nload; i++) {
+    const int f = fld(i);
+    if (PIPE == 1 && i + 1 < nload) load(nx, fld(i + 1));
+    if (PIPE == 2 && i + 2 < nload) load(nx2, fld(i + 2));
+    if constexpr (CURLX) {
+      const R kz = (R)k;
+      if (i == 1) {
+#pragma unroll
+        for (int c = 0; c < E; c++) ox[c] = mk<R>(kz * v[c].x, kz * v[c].y);
+      } else if (i == 2) {
+#pragma unroll
+        for (int c = 0; c < E; c++) {  // I*(ky w - kz v), I*z = (-Im z, Re z)
+          const R ky = (R)wave<N>(t + TPP * c);
+          ox[c] = mk<R>(-(ky * v[c].y - ox[c].y), ky * v[c].x - ox[c].x);
+        }
+      }
+    }
+    fft_store(v, f);
+    if (CURLX && i == 2) fft_store(ox, 3);
+    if (i + 1 < nload) {
       if (PIPE) {
 #pragma unroll
         for (int c = 0; c < E; c++) {
@@ -127,7 +157,7 @@ __global__ __launch_bounds__(T *(N / E), MINW) void k_ypass(cx<R> *__restrict__ 
           if (PIPE == 2) nx[c] = nx2[c];
         }
       } else {
-        load(v, f + 1);
+        load(v, fld(i + 1));
       }
     }
   }
@@ -165,13 +195,17 @@ __global__ __launch_bounds__(T *(N / Sched<N>::E)) void k_xplain(const cx<RI> *i
 }
 
 // ---------------------------------------------------------------------------
-// A: curl prologue + inverse FFT along x.
-// Writes omega partials to W fields 3,4,5 and, if WITH_U, the velocity
-// partials to W fields 0,1,2.  The curl follows main.c:359-361:
+// A: curl prologue + inverse FFT along x.  The curl follows main.c:359-361:
 //   curlZ = I*(kx V - ky U), curlY = I*(kz U - kx W), curlX = I*(ky W - kz V)
-// with I*z = (-Im z, Re z).  Blocks own x-pencil tiles: grid (ceil(nf/T), n),
-// blockIdx.y = j.  U, V, Wv hold the spectral velocity at the thread's
-// positions i = t + TPP*c of pencil (j, k).
+// with I*z = (-Im z, Re z).
+// WITH_U = false: writes the three omega partials to W fields 3,4,5.
+// WITH_U = true : writes u, v, w to fields 0,1,2 and curlY, curlZ to 4,5.
+//   curlX has no kx factor, so its inverse x transform is I*(ky w - kz v) of
+//   the transformed v and w: the inverse y pass forms it from fields 1 and 2
+//   (k_ypass CURLX) and this pass neither transforms nor stores it.
+// Blocks own x-pencil tiles: grid (ceil(nf/T), n), blockIdx.y = j.  U, V, Wv
+// hold the spectral velocity at the thread's positions i = t + TPP*c of
+// pencil (j, k).
 template <int N, typename R, int T, bool WITH_U, int E>
 __device__ __forceinline__ void curl_inverse_x(cx<R> (&U)[E], cx<R> (&V)[E], cx<R> (&Wv)[E],
                                                cx<R> *__restrict__ W, Geo g, int t, int p, int j, int k, bool ok,
@@ -189,10 +223,12 @@ __device__ __forceinline__ void curl_inverse_x(cx<R> (&U)[E], cx<R> (&V)[E], cx<
   auto irot = [](R a, R b) { return mk<R>(-b, a); };  // I*(a + i b)
   {
     cx<R> v[E];
+    if constexpr (!WITH_U) {
 #pragma unroll
-    for (int c = 0; c < E; c++) v[c] = irot(ky * Wv[c].x - kz * V[c].x, ky * Wv[c].y - kz * V[c].y);  // curlX
-    fft_pencil<N, R, true>(v, t, lds, slot, tw);
-    store(3, v);
+      for (int c = 0; c < E; c++) v[c] = irot(ky * Wv[c].x - kz * V[c].x, ky * Wv[c].y - kz * V[c].y);  // curlX
+      fft_pencil<N, R, true>(v, t, lds, slot, tw);
+      store(3, v);
+    }
 #pragma unroll
     for (int c = 0; c < E; c++) {  // curlY = I*(kz U - kx W)
       const R kx = (R)wave<N>(t + TPP * c);
