@@ -187,8 +187,11 @@ def validate(a):
     FP64, run as one batch and compared line by line with the reference's
     own trajectories (directory 0256 of the reference, excerpted in
     tests/golden/tgv_trajectories.json); also the dissipation rate
-    eps = 2 Z / Re (tgv.gp:9-11) and its peak."""
+    eps = 2 Z / Re (tgv.gp:11), its peak, and the match of the six eps
+    curves with the digitised Brachet et al. points the reference plots
+    them over (img/ref.txt, in tests/golden/brachet_dissipation.json)."""
     import dns_amd
+    from dns_amd import dissipation as dis
     with open(os.path.join(ROOT, "tests", "golden", "tgv_trajectories.json")) as f:
         gold = json.load(f)
     res = [100, 200, 400, 800, 1600, 3000]
@@ -202,11 +205,14 @@ def validate(a):
         b.load_velocity(m, u)
     out = {r: {"max_rel_E": 0.0, "max_rel_Z": 0.0, "first_step_over_1e-12": None, "peak_eps": 0.0,
                "peak_eps_t": 0.0, "gold_peak_eps": 0.0} for r in res}
+    curves = {r: ([], []) for r in res}
     t0 = time.perf_counter()
     step = 0
     while step <= a.val_steps:
         for m, r in enumerate(res):
             e, z = b.diagnostics(m)
+            curves[r][0].append(step * dt)
+            curves[r][1].append(2 * z / r)
             o = out[r]
             if step in rows[m]:
                 ge, gz = rows[m][step]
@@ -227,10 +233,13 @@ def validate(a):
             print(f"validate: step {step}", file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
     b.close()
+    brachet = dis.match(dis.load_points(os.path.join(ROOT, "tests", "golden", "brachet_dissipation.json")),
+                        {r: (np.array(c[0]), np.array(c[1])) for r, c in curves.items()})["summary"]
     line = {"metric": "Taylor-Green n=256 trajectories vs the reference's golden runs", "value":
             max(max(o["max_rel_E"], o["max_rel_Z"]) for o in out.values()), "unit": "max relative deviation",
             "higher_is_better": False, "n_gpus": 1, "dtype": "f64", "steps": a.val_steps, "wall_s": wall,
-            "per_Re": out, "config": {"workload": "TGV n=256, dt=0.0025, nu=1/Re", "Re": res}}
+            "per_Re": out, "brachet_eps_match": brachet,
+            "config": {"workload": "TGV n=256, dt=0.0025, nu=1/Re", "Re": res}}
     print(json.dumps(line), flush=True)
     return 0
 

@@ -7,6 +7,11 @@ run.sh:1-10 and hal.sh:1-16).  This script keeps, per file, the lines of
 steps 0..400, every 100th step after that, and the final line, as data
 (step, t, energy, enstrophy) -- no reference source is copied.
 
+It also copies the reference's digitised dissipation-rate points (img/ref.txt:
+(t, eps) pairs read off the Taylor-Green curves of Brachet et al. 1983 for
+Re = 100..3000, plotted by tgv.gp against the runs' 2 Z / Re) into
+brachet_dissipation.json, sorted by time.
+
 Run in the development container (the reference is not on the GPU box):
     python tests/golden/make_golden.py /root/reference
 """
@@ -39,6 +44,19 @@ def main(root):
             f.write("]}%s\n" % ("," if ki < len(out) - 1 else ""))
         f.write("}\n")
     print(path, sum(len(v["rows"]) for v in out.values()), "rows")
+    pts = []
+    with open(os.path.join(root, "img", "ref.txt")) as f:
+        for line in f:
+            if line.strip():
+                t, e = line.split()
+                pts.append([t, e])
+    pts.sort(key=lambda r: float(r[0]))
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "brachet_dissipation.json")
+    with open(path, "w") as f:
+        f.write('{"source": "img/ref.txt", "columns": ["t", "eps"], "points": [\n')
+        f.write(",\n".join(json.dumps(r) for r in pts))
+        f.write("]}\n")
+    print(path, len(pts), "points")
 
 
 if __name__ == "__main__":
