@@ -518,7 +518,9 @@ __global__ __launch_bounds__(PPW *(N / Sched<N>::E)) void k_zc2r(const cx<R> *__
 // accurate anyway, being made from the float nonlinear term); stage 3 forms
 // (Un + S) + a3 dU3 in double and also writes its float copy, which stages
 // 0..2 of the next step read (whoever sets Un from outside refreshes it).
-// The epilogue runs in double whatever R is.  phat (optional) receives P.
+// The epilogue runs in double, except in mixed stages 0..2, whose inputs and
+// outputs are all float and which run it in float.  phat (optional, stage
+// 3) receives P.
 // The new state (Ur' for stages 0..2, Un for stage 3) is returned in
 // U,V,Wv as the next stage's pass A would read it back from memory.
 // Grid (ceil(nf/T), n): blockIdx.y = j.
@@ -575,6 +577,7 @@ __device__ __forceinline__ void forward_x_rk(const cx<R> *__restrict__ W, cx<SR>
   const double ky = (double)wave<N>(j), kz = (double)k;
   const double nudt = nu * dt;
   const bool myz = fabs(ky) < q.kmax && fabs(kz) < q.kmax;
+  const float nudtf = (float)nudt, af = (float)q.a, bf = (float)q.b;  // mixed stages 0..2
   // The state values of a batch of positions (all three fields) are loaded
   // together and the next batch's loads are issued before this batch's
   // stores: the stores may alias the loads as far as the compiler can tell,
@@ -624,6 +627,43 @@ __device__ __forceinline__ void forward_x_rk(const cx<R> *__restrict__ W, cx<SR>
       const double kx = (double)wave<N>(t + TPP * c);
       const double kk = kx * kx + ky * ky + kz * kz;
       const double fac = (myz && fabs(kx) < q.kmax) ? dt : 0.0;
+      if constexpr (MIXED_STATE && STAGE < 3) {
+        // Stages 0..2 of the mixed state: every input (F, the float copy of
+        // Un, S) and every output (Ur', S) is a float, so the expressions of
+        // main.c:381-413 are evaluated in float (same order); wavenumbers and
+        // kk are exact integers in float.
+        const float kxf = (float)kx, kkf = (float)kk, facf = (float)fac;
+        const float kvf[3] = {kxf, (float)ky, (float)kz};
+        float2 e[3];
+#pragma unroll
+        for (int f = 0; f < 3; f++) e[f] = make_float2((float)F[f][c].x * facf, (float)F[f][c].y * facf);
+        float2 Pf = make_float2(0.f, 0.f);
+        if (kkf > 0) {
+          Pf.x = (e[0].x * kvf[0] + e[1].x * kvf[1] + e[2].x * kvf[2]) / kkf;
+          Pf.y = (e[0].y * kvf[0] + e[1].y * kvf[1] + e[2].y * kvf[2]) / kkf;
+        }
+        const float vkf = nudtf * kkf;
+        cx<R> nxt[3];
+#pragma unroll
+        for (int f = 0; f < 3; f++) {
+          const long lf = f * g.fs + l;
+          const float2 ur = cvt<float, URS>(B.ur[f][i]);
+          e[f].x -= Pf.x * kvf[f] + vkf * ur.x;
+          e[f].y -= Pf.y * kvf[f] + vkf * ur.y;
+          const float2 du = e[f];
+          const float2 un = STAGE == 0 ? ur : B.unf[f][i];
+          const float2 nr = make_float2(un.x + bf * du.x, un.y + bf * du.y);
+          st<SR>(uhr + lf, cvt<SR, float>(nr));
+          const float2 sa = STAGE == 0 ? make_float2(0.f, 0.f) : B.sf[f][i];
+          st<float>(inc + 3 * g.fs + lf,
+                    STAGE == 0 ? make_float2(af * du.x, af * du.y) : make_float2(sa.x + af * du.x, sa.y + af * du.y));
+          nxt[f] = cvt<R, float>(nr);
+        }
+        U[c] = nxt[0];
+        V[c] = nxt[1];
+        Wv[c] = nxt[2];
+        continue;
+      }
       double2 d[3];
 #pragma unroll
       for (int f = 0; f < 3; f++) d[f] = make_double2((double)F[f][c].x * fac, (double)F[f][c].y * fac);
