@@ -10,7 +10,9 @@
  *   DNS_DEVICE=<ordinal>       HIP device (default 0)
  *
  * Differences that cannot be avoided: under -v the reference prints
- * "dns: omp_get_max_threads: N"; this prints "dns: device: <name>".
+ * "dns: omp_get_max_threads: N"; this prints "dns: device: <ordinal>",
+ * and at the end "dns: <steps> steps in <s> s (<steps/s> steps/s)" (wall
+ * time of the step loop, diagnostics and dumps included).
  * The reference dies with SIGFPE at the first overflow/invalid operation
  * (main.c:49-50); here the diagnostics are checked every 10 steps and a
  * non-finite energy or enstrophy raises SIGFPE before its line is printed.
@@ -21,6 +23,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 #include "dns.h"
 
 enum { nvars = 4 };
@@ -65,6 +68,12 @@ static void fail(int rc) {
   else
     fprintf(stderr, "dns: error: %s\n", dns_strerror(rc));
   exit(1);
+}
+
+static double wall_seconds(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
 }
 
 static void nonfinite(void) {
@@ -239,6 +248,7 @@ int main(int argc, char **argv) {
     fail(rc);
   double dx = 2 * pi / n, t = 0.0;
   long tstep = 0, idump = 0, pending = 0;
+  const double t_start = wall_seconds();
   for (;;) { /* main.c:256-423; steps are queued and run before each line */
     if (tstep % 10 == 0) {
       double energy, Omega;
@@ -280,6 +290,10 @@ int main(int argc, char **argv) {
     rc = dns_diagnostics(ctx, &e, &z);
     if (rc == DNS_ENONFINITE)
       nonfinite();
+  }
+  if (Verbose) {
+    const double el = wall_seconds() - t_start;
+    fprintf(stderr, "dns: %ld steps in %.3f s (%.2f steps/s)\n", tstep, el, el > 0 ? tstep / el : 0.0);
   }
   free(buf);
   dns_destroy(ctx);

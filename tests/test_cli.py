@@ -3,12 +3,13 @@ binary and the CPU oracle `dns_cpu` must give the reference's messages and
 exit codes for every argument/file error, without touching a GPU.  Also
 checks the Taylor-Green generator `dns-ic` against tgv.py's closed form."""
 import os
+import signal
 import subprocess
 
 import numpy as np
 import pytest
 
-from conftest import DNS_CLI, DNS_IC, ORACLE_CLI, tgv_field
+from conftest import DNS_CLI, DNS_IC, ORACLE_CLI, tgv_field, write_raw
 
 CASES = [
     ([], "dns: error: -t is not set or invalid\n"),
@@ -74,3 +75,29 @@ def test_dns_ic_level(tmp_path):
     r = subprocess.run([DNS_IC, "-l", "3", "-o", str(p)], check=True, capture_output=True, text=True)
     assert os.path.getsize(p) == 4 * 8 ** 3 * 8
     assert "n=8" in r.stderr
+
+
+def test_oracle_traps_nonfinite_input(tmp_path):
+    """main.c:49-50 enables FP traps: an Inf in the input makes the initial
+    r2c compute inf - inf, so the process dies with SIGFPE before its first
+    line.  (The GPU binary reproduces this from its non-finite check;
+    tests/test_gpu_cli.py compares the two.)"""
+    u = tgv_field(16)
+    u[0, 3, 4, 5] = np.inf
+    p = tmp_path / "inf.raw"
+    write_raw(p, u[:3])
+    r = subprocess.run([ORACLE_CLI, "-i", str(p), "-n", "0.001", "-s", "0.01", "-t", "1"],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == -signal.SIGFPE
+    assert r.stdout == ""
+
+
+def test_oracle_traps_overflow(tmp_path):
+    """A violently unstable time step overflows within a few steps: the
+    line of step 0 is printed, then SIGFPE."""
+    p = tmp_path / "tgv.raw"
+    write_raw(p, tgv_field(16)[:3])
+    r = subprocess.run([ORACLE_CLI, "-i", str(p), "-n", "0.001", "-s", "5", "-t", "1000"],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == -signal.SIGFPE
+    assert r.stdout == "         0  0.0000000000000000e+00  1.2500000000000000e-01  3.7500000000000000e-01\n"

@@ -5,6 +5,7 @@ XDMF files), and the isotropic initial condition of `dns-ic`.
 Tolerance: FP64 energy/enstrophy within 1e-12 relative of the oracle over
 the whole run; snapshot fields within 1e-12 of max."""
 import os
+import signal
 import subprocess
 
 import numpy as np
@@ -136,3 +137,76 @@ def test_isotropic_run_matches_oracle(tmp_path, oracle):
         oe, oz = o.diagnostics()
         assert abs(ge - oe) / oe < 1e-12 and abs(gz - oz) / oz < 1e-12
     g.close()
+
+
+def _run_any(exe, args, cwd):
+    return subprocess.run([exe, *args], cwd=cwd, capture_output=True, text=True, timeout=600)
+
+
+def test_nonfinite_input_raises_sigfpe(tmp_path):
+    """An Inf in the input: the reference (FP traps, main.c:49-50) and the
+    oracle die with SIGFPE before the first line; dns detects the
+    non-finite energy of step 0 and raises SIGFPE itself."""
+    u = tgv_field(16)
+    u[0, 3, 4, 5] = np.inf
+    p = tmp_path / "inf.raw"
+    write_raw(p, u[:3])
+    args = ["-i", str(p), "-n", "0.001", "-s", "0.01", "-t", "1"]
+    print("nonfinite input: dns and dns_cpu", flush=True)
+    g, o = _run_any(DNS_CLI, args, tmp_path), _run_any(ORACLE_CLI, args, tmp_path)
+    assert o.returncode == g.returncode == -signal.SIGFPE
+    assert g.stdout == o.stdout == ""
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_unstable_run_raises_sigfpe(tmp_path, precision):
+    """dt = 5 at n=16 overflows by step 3: the same stdout (the line of
+    step 0) and the same signal as the oracle, whose traps fire at the
+    first overflow; dns finds the non-finite state at the next line."""
+    p = tmp_path / "tgv.raw"
+    write_raw(p, tgv_field(16)[:3])
+    args = ["-i", str(p), "-n", "0.001", "-s", "5", "-t", "1000"]
+    print(f"unstable run: dns ({precision}) and dns_cpu", flush=True)
+    env = dict(os.environ, DNS_PRECISION=precision)
+    g = subprocess.run([DNS_CLI, *args], cwd=tmp_path, capture_output=True, text=True, timeout=600, env=env)
+    o = _run_any(ORACLE_CLI, args, tmp_path)
+    assert o.returncode == g.returncode == -signal.SIGFPE
+    assert g.stdout == o.stdout
+    assert len(g.stdout.splitlines()) == 1
+
+
+@pytest.mark.skipif(hasattr(os, "geteuid") and os.geteuid() == 0, reason="root may write a read-only directory")
+def test_unwritable_dump_directory(tmp_path):
+    """-d in a directory the process cannot write: dns reports the file it
+    could not open and exits 1 (the reference does not check this fopen)."""
+    p = tmp_path / "tgv.raw"
+    write_raw(p, tgv_field(16)[:3])
+    ro = tmp_path / "ro"
+    ro.mkdir()
+    os.chmod(ro, 0o555)
+    try:
+        print("unwritable dump directory: dns", flush=True)
+        r = _run_any(DNS_CLI, ["-d", "-i", str(p), "-n", "0.01", "-s", "0.01", "-t", "0.05"], ro)
+    finally:
+        os.chmod(ro, 0o755)
+    assert r.returncode == 1
+    assert r.stderr == "dns: error: fail to open '00000000.raw'\n"
+    assert r.stdout.splitlines()[0].split()[0] == "0"
+
+
+def test_verbose_lines(tmp_path):
+    """-v: "dns: device: <ordinal>" (in place of the reference's thread
+    count) and "dns: n = <n>" (main.c:159-160,172-173), then the step loop's
+    wall time; stdout is unchanged by -v."""
+    p = tmp_path / "tgv.raw"
+    write_raw(p, tgv_field(16)[:3])
+    args = ["-i", str(p), "-n", "0.01", "-s", "0.01", "-t", "0.195"]
+    print("verbose: dns -v", flush=True)
+    v = _run(DNS_CLI, ["-v", *args], tmp_path)
+    q = _run(DNS_CLI, args, tmp_path)
+    assert v.stdout == q.stdout and len(v.stdout.splitlines()) == 3
+    lines = v.stderr.splitlines()
+    assert lines[0] == "dns: device: 0" and lines[1] == "dns: n = 16"
+    head, _, rest = lines[2].partition(" steps in ")
+    assert head == "dns: 20" and rest.endswith(" steps/s)"), lines[2]
+    assert q.stderr == ""
