@@ -210,6 +210,10 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
   // threads and one leaves a small spill (1.14 against 1.13 ms at n=512);
   // in double one ahead costs 192 against 160 VGPRs (2.51 against 2.47 ms)
   static constexpr int PIPE_YC = 0;
+  // curlX formed by the inverse y pass, except where that pass runs 16
+  // double elements per thread (n=1024): there its extra array spills
+  // (348 bytes; 16.1 -> 21.5 ms), so the x pass transforms and stores curlX
+  static constexpr bool CX_IN_Y = !(sizeof(R) == 8 && E_Y > E);
   // plain transforms of the working precision (rfft3 / irfft3)
   static constexpr int T_P = sizeof(R) == 8 ? T_D : (16 * TPP > 1024 ? 1024 / TPP : 16);
   static constexpr int E_X = (sizeof(R) == 4 && E == 8 && N >= 256) ? 16 : E;
@@ -425,7 +429,10 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
           DNS_TIMED(c, sl + 1, (k_ypass<N, R, TY, true, 1, PIPE_Y, false, E_Y><<<gy, by, 0, s>>>(W + 3 * g.fs, 3, g, tw, g.mW)));
           DNS_TIMED(c, sl + 2, (k_zfused<N, R, PP, 0, R><<<gz, bz, 0, s>>>(W, (R *)c->us, g, invn3, tw)));
         } else {
-          DNS_TIMED(c, sl + 1, (k_ypass<N, R, TY, true, 1, PIPE_YC, false, E_Y, true><<<gy, by, 0, s>>>(W, 6, g, tw, g.mW)));
+          if constexpr (CX_IN_Y)
+            DNS_TIMED(c, sl + 1, (k_ypass<N, R, TY, true, 1, PIPE_YC, false, E_Y, true><<<gy, by, 0, s>>>(W, 6, g, tw, g.mW)));
+          else
+            DNS_TIMED(c, sl + 1, (k_ypass<N, R, TY, true, 1, PIPE_Y, false, E_Y><<<gy, by, 0, s>>>(W, 6, g, tw, g.mW)));
           if (rk < 3)
             DNS_TIMED(c, sl + 2, (k_zfused<N, R, PP, 1, R><<<gz, bz, 0, s>>>(W, (R *)c->us, g, invn3, tw)));
           else
@@ -436,16 +443,16 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
         // E fused with the next stage's A (the next step's stage 1 after rk 3)
         switch (rk) {
           case 0:
-            DNS_TIMED(c, sl + 5, (k_xfwd_rk_xinv<N, R, TX, 0, SR, 1, EPI_X, E_X><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, nullptr, g, q, tw)));
+            DNS_TIMED(c, sl + 5, (k_xfwd_rk_xinv<N, R, TX, 0, SR, 1, EPI_X, E_X, !CX_IN_Y><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, nullptr, g, q, tw)));
             break;
           case 1:
-            DNS_TIMED(c, sl + 5, (k_xfwd_rk_xinv<N, R, TX, 1, SR, 1, EPI_X, E_X><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, nullptr, g, q, tw)));
+            DNS_TIMED(c, sl + 5, (k_xfwd_rk_xinv<N, R, TX, 1, SR, 1, EPI_X, E_X, !CX_IN_Y><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, nullptr, g, q, tw)));
             break;
           case 2:
-            DNS_TIMED(c, sl + 5, (k_xfwd_rk_xinv<N, R, TX, 2, SR, 1, EPI_X, E_X><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, nullptr, g, q, tw)));
+            DNS_TIMED(c, sl + 5, (k_xfwd_rk_xinv<N, R, TX, 2, SR, 1, EPI_X, E_X, !CX_IN_Y><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, nullptr, g, q, tw)));
             break;
           default:
-            DNS_TIMED(c, sl + 5, (k_xfwd_rk_xinv<N, R, TX, 3, SR, 1, 0, E_X><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, ph, g, q, tw)));
+            DNS_TIMED(c, sl + 5, (k_xfwd_rk_xinv<N, R, TX, 3, SR, 1, 0, E_X, !CX_IN_Y><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, ph, g, q, tw)));
             break;
         }
         if (rk < 3 && c->rec[rk])

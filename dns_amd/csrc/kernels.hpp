@@ -202,11 +202,12 @@ __global__ __launch_bounds__(T *(N / Sched<N>::E)) void k_xplain(const cx<RI> *i
 // WITH_U = true : writes u, v, w to fields 0,1,2 and curlY, curlZ to 4,5.
 //   curlX has no kx factor, so its inverse x transform is I*(ky w - kz v) of
 //   the transformed v and w: the inverse y pass forms it from fields 1 and 2
-//   (k_ypass CURLX) and this pass neither transforms nor stores it.
+//   (k_ypass CURLX) and this pass neither transforms nor stores it, unless
+//   CX (shapes whose y pass has no registers for it) asks for field 3 too.
 // Blocks own x-pencil tiles: grid (ceil(nf/T), n), blockIdx.y = j.  U, V, Wv
 // hold the spectral velocity at the thread's positions i = t + TPP*c of
 // pencil (j, k).
-template <int N, typename R, int T, bool WITH_U, int E>
+template <int N, typename R, int T, bool WITH_U, int E, bool CX = false>
 __device__ __forceinline__ void curl_inverse_x(cx<R> (&U)[E], cx<R> (&V)[E], cx<R> (&Wv)[E],
                                                cx<R> *__restrict__ W, Geo g, int t, int p, int j, int k, bool ok,
                                                cx<R> *lds, const cx<R> *__restrict__ tw) {
@@ -223,7 +224,7 @@ __device__ __forceinline__ void curl_inverse_x(cx<R> (&U)[E], cx<R> (&V)[E], cx<
   auto irot = [](R a, R b) { return mk<R>(-b, a); };  // I*(a + i b)
   {
     cx<R> v[E];
-    if constexpr (!WITH_U) {
+    if constexpr (!WITH_U || CX) {
 #pragma unroll
       for (int c = 0; c < E; c++) v[c] = irot(ky * Wv[c].x - kz * V[c].x, ky * Wv[c].y - kz * V[c].y);  // curlX
       fft_pencil<N, R, true>(v, t, lds, slot, tw);
@@ -753,7 +754,8 @@ __device__ __forceinline__ void forward_x_rk(const cx<R> *__restrict__ W, cx<SR>
 // update straight into the curl prologue and inverse x transform, saving
 // the re-read of the state.  W fields 0..2 (this stage's N) are read before
 // the same pencils' fields 0..5 are overwritten with the next partials.
-template <int N, typename R, int T, int STAGE, typename SR, int MINW = 1, int EPI = 0, int E = Sched<N>::E>
+template <int N, typename R, int T, int STAGE, typename SR, int MINW = 1, int EPI = 0, int E = Sched<N>::E,
+          bool CX = false>
 __global__ __launch_bounds__(T *(N / E), MINW) void k_xfwd_rk_xinv(cx<R> *__restrict__ W,
                                                                        cx<SR> *__restrict__ uhr,
                                                                        double2 *__restrict__ uhn,
@@ -768,7 +770,7 @@ __global__ __launch_bounds__(T *(N / E), MINW) void k_xfwd_rk_xinv(cx<R> *__rest
   const bool ok = k < N / 2 + 1;
   cx<R> U[E], V[E], Wv[E];
   forward_x_rk<N, R, T, STAGE, SR, EPI, E>(W, uhr, uhn, acc, phat, g, q, tw, t, p, j, k, ok, lds, U, V, Wv);
-  curl_inverse_x<N, R, T, (STAGE < 3), E>(U, V, Wv, W + blockIdx.z * g.mW, g, t, p, j, k, ok, lds, tw);
+  curl_inverse_x<N, R, T, (STAGE < 3), E, CX>(U, V, Wv, W + blockIdx.z * g.mW, g, t, p, j, k, ok, lds, tw);
 }
 
 // ---------------------------------------------------------------------------

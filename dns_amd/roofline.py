@@ -17,7 +17,9 @@ spectral field and Rr = n^3 reals per physical field:
 where Zc = n^2 nk and Yc = n (2nk-1) nk count the dealiased part of the
 nonlinear term (nk = #{m >= 0 : m < kmax}, kmax = 2/3 (n/2+1)).  From stage 2
 on, A hands u, v, w, omega_y, omega_z to B, which forms omega_x from v and w
-(5 fields read, 6 written).
+(5 fields read, 6 written) -- except in FP64 where a y pencil needs more
+than 64 threads at the schedule's elements per thread (n=1024): there A
+writes all six fields and B reads six.
 
 Partials (A..E outputs) and the stale velocity are in the working precision
 R; the step-start state and the accumulator are always double; SR is the
@@ -38,6 +40,12 @@ def n_box(n):
     return m
 
 
+def curlx_in_y(n, precision="fp64"):
+    """Whether the inverse y pass forms omega_x (see the table above)."""
+    e = 12 if n % 3 == 0 else (4 if n == 16 else 8)   # elements per thread of the schedule
+    return not (precision == "fp64" and n // e > 64)
+
+
 def slot_bytes(n, precision="fp64"):
     """{slot name: algorithmic bytes of one launch}; slot = pass.s<stage>.
 
@@ -52,9 +60,10 @@ def slot_bytes(n, precision="fp64"):
     r = 8 if precision == "fp64" else 4      # real scalar in the working precision
     cr, csr, c64 = 2 * r, 2 * r, 16
     out = {}
+    ax = 5 if curlx_in_y(n, precision) else 6     # fields A writes from stage 2 on
     for st in range(4):
-        a = 3 * uc * c64 + 3 * uc * cr if st == 0 else 3 * uc * csr + 5 * uc * cr
-        b = (3 + 3 if st == 0 else 5 + 6) * uc * cr
+        a = 3 * uc * c64 + 3 * uc * cr if st == 0 else 3 * uc * csr + ax * uc * cr
+        b = (3 + 3 if st == 0 else ax + 6) * uc * cr
         if st == 0:
             c = 3 * uc * cr + 3 * rr * r + 3 * zc * cr
         elif st < 3:
@@ -78,7 +87,7 @@ def slot_bytes(n, precision="fp64"):
             else:          # r Ur, Un(f64), S; w Un+1(f64) and its float copy
                 e = 3 * yc * cr + 3 * uc * csr + 3 * uc * c64 + 3 * uc * c32 + 3 * uc * c64 + 3 * uc * c32
         # E fused with the next stage's A: A's writes without its read
-        nxt_writes = 3 * uc * cr if st == 3 else 5 * uc * cr
+        nxt_writes = 3 * uc * cr if st == 3 else ax * uc * cr
         for p, v in zip(PASSES, (a, b, c, d, e, e + nxt_writes)):
             out[f"{p}.s{st + 1}"] = v
     return out
