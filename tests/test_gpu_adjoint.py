@@ -98,5 +98,5 @@ def test_torch_autograd_wrapper():
         x = torch.tensor(u0, dtype=torch.float64, device=dev, requires_grad=True)
         J = final_enstrophy(x, nu, dt, nsteps, every=3)
         (2.5 * J).backward()
-        assert abs(float(J) - J0) < 1e-14 * J0
+        assert abs(J.item() - J0) < 1e-14 * J0
         assert rel(x.grad.cpu().numpy(), 2.5 * g0) < 1e-14

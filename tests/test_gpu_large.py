@@ -1,0 +1,118 @@
+"""The large grids' own kernel instances on the GPU.
+
+Every grid size compiles its own launch shapes: 16-column float tiles with
+16 elements per thread and four-position epilogue batches from n=256, the
+radix-3 schedules of 192, 384 and 768, the two-wave z pencils of n=1024.
+The small-n parity tests do not run these.  Here: FP64 steps against the
+CPU oracle up to n=512, mixed against FP64 on the same grids, and the
+Beltrami closed form at n=768 (the oracle would take minutes there).  The
+n=1024 instance is checked by `bench.py --n 1024`, whose FP64 and mixed
+runs (different curl paths at that size) agree in energy to ~4e-13.
+
+Inputs are sums of separable Fourier modes (cheap to build at 512^3),
+including modes beyond the 2/3 cutoff and on the Nyquist planes."""
+import numpy as np
+import pytest
+
+import dns_amd
+from conftest import beltrami_field, rel, rk4_amplification
+
+pytestmark = pytest.mark.gpu
+
+
+def modes_field(n, seed, nmodes=6):
+    """Velocity made of `nmodes` separable modes per component,
+    A sin|cos(kx x) sin|cos(ky y) sin|cos(kz z), wavenumbers up to n/2."""
+    rng = np.random.default_rng(seed)
+    x = 2 * np.pi * np.arange(n) / n
+    u = np.zeros((3, n, n, n))
+    for c in range(3):
+        for m in range(nmodes):
+            # the first mode of each component sits on the Nyquist plane of one
+            # axis, the second beyond the 2/3 cutoff, the rest anywhere
+            k = rng.integers(0, n // 3, size=3)
+            if m == 0:
+                k[c] = n // 2
+            elif m == 1:
+                k[(c + 1) % 3] = n // 2 - 2
+            f = [np.cos(k[a] * x) if rng.integers(2) else np.sin(k[a] * x) for a in range(3)]
+            if not np.any(f[0]) or not np.any(f[1]) or not np.any(f[2]):
+                continue  # sin of a Nyquist wavenumber vanishes on the grid
+            u[c] += rng.uniform(0.5, 1.0) * f[0][:, None, None] * f[1][None, :, None] * f[2][None, None, :]
+    return u
+
+
+@pytest.mark.parametrize("n", [128, 192, 256, 384, 512])
+def test_fp64_steps_match_oracle(oracle, n):
+    nu, dt = 1.0 / 400, 1e-3
+    print(f"fp64 vs oracle n={n}", flush=True)
+    u = modes_field(n, seed=n)
+    with dns_amd.Solver(n, nu, dt, "fp64") as g:
+        o = oracle.solver(n, nu, dt)
+        try:
+            g.load_velocity(u)
+            o.load_velocity(u)
+            del u
+            g.step(2)
+            o.step(2)
+            assert rel(g.get_spectral(), o.get_spectral()) < 1e-12
+            assert rel(g.get_stale_velocity(), o.get_stale_velocity()) < 1e-12
+            ge, gz = g.diagnostics()
+            oe, oz = o.diagnostics()
+            assert abs(ge - oe) / oe < 1e-13 and abs(gz - oz) / oz < 1e-13
+        finally:
+            o.close()
+
+
+@pytest.mark.parametrize("n", [256, 384, 512])
+def test_mixed_tracks_fp64(n):
+    nu, dt = 1.0 / 400, 1e-3
+    print(f"mixed vs fp64 n={n}", flush=True)
+    u = modes_field(n, seed=7 * n)
+    with dns_amd.Solver(n, nu, dt, "fp64") as a, dns_amd.Solver(n, nu, dt, "mixed") as b:
+        a.load_velocity(u)
+        b.load_velocity(u)
+        del u
+        for _ in range(2):
+            a.step(5)
+            b.step(5)
+            ea, za = a.diagnostics()
+            eb, zb = b.diagnostics()
+            assert abs(eb - ea) / ea < 1e-6 and abs(zb - za) / za < 1e-5
+        assert rel(b.get_spectral(), a.get_spectral()) < 1e-5
+        # the physical velocity is stored in float and this field carries
+        # modes up to the Nyquist planes, where float rounding of the
+        # transforms weighs most (n=384: 1.1e-5 of max |u| after 10 steps)
+        assert rel(b.get_stale_velocity(), a.get_stale_velocity()) < 5e-5
+
+
+def beltrami_broadcast(n, k, a=1.0, b=0.7, c=0.4):
+    """The ABC field of conftest.beltrami_field built from 1-D factors."""
+    x = 2 * np.pi * np.arange(n) / n
+    s, co = np.sin(k * x), np.cos(k * x)
+    u = np.empty((3, n, n, n))
+    u[0] = a * s[None, None, :] + c * co[None, :, None]
+    u[1] = b * s[:, None, None] + a * co[None, None, :]
+    u[2] = c * s[None, :, None] + b * co[:, None, None]
+    return u
+
+
+def test_beltrami_broadcast_is_beltrami_field():
+    assert np.array_equal(beltrami_broadcast(12, 2), beltrami_field(12, 2))
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("mixed", 1e-6)])
+def test_beltrami_n768(precision, tol):
+    """curl u = k u, so u x omega = 0 and every mode decays by the RK4
+    polynomial R = R(nu dt k^2) per step: energy and enstrophy (computed on
+    the device) scale by R^2 per step on the radix-3 grid n=768."""
+    n, nu, dt, k, m = 768, 0.002, 0.01, 7, 3
+    print(f"beltrami {precision} n={n}", flush=True)
+    with dns_amd.Solver(n, nu, dt, precision) as s:
+        s.load_velocity(beltrami_broadcast(n, k))
+        e0, z0 = s.diagnostics()
+        assert abs(z0 - k * k * e0) <= 1e-13 * z0   # a single shell |k| = k
+        s.step(m)
+        e, z = s.diagnostics()
+        f = rk4_amplification(nu * dt * k * k) ** (2 * m)
+        assert abs(e - e0 * f) / (e0 * f) < tol and abs(z - z0 * f) / (z0 * f) < tol
