@@ -519,9 +519,10 @@ __global__ __launch_bounds__(PPW *(N / Sched<N>::E)) void k_zc2r(const cx<R> *__
 // accurate anyway, being made from the float nonlinear term); stage 3 forms
 // (Un + S) + a3 dU3 in double and also writes its float copy, which stages
 // 0..2 of the next step read (whoever sets Un from outside refreshes it).
-// The epilogue runs in double, except in mixed stages 0..2, whose inputs and
-// outputs are all float and which run it in float.  phat (optional, stage
-// 3) receives P.
+// The epilogue runs in double for the FP64 state; for the mixed state dU
+// and P are evaluated in float (their inputs are float) and only the stage
+// 3 sum Un + S + a3 dU is formed in double.  phat (optional, stage 3)
+// receives P.
 // The new state (Ur' for stages 0..2, Un for stage 3) is returned in
 // U,V,Wv as the next stage's pass A would read it back from memory.
 // Grid (ceil(nf/T), n): blockIdx.y = j.
@@ -628,11 +629,12 @@ __device__ __forceinline__ void forward_x_rk(const cx<R> *__restrict__ W, cx<SR>
       const double kx = (double)wave<N>(t + TPP * c);
       const double kk = kx * kx + ky * ky + kz * kz;
       const double fac = (myz && fabs(kx) < q.kmax) ? dt : 0.0;
-      if constexpr (MIXED_STATE && STAGE < 3) {
-        // Stages 0..2 of the mixed state: every input (F, the float copy of
-        // Un, S) and every output (Ur', S) is a float, so the expressions of
-        // main.c:381-413 are evaluated in float (same order); wavenumbers and
-        // kk are exact integers in float.
+      if constexpr (MIXED_STATE) {
+        // Mixed state: dU is made from the float nonlinear term and the float
+        // stage state, so the expressions of main.c:381-399 are evaluated in
+        // float (same order; wavenumbers and kk are exact integers in float).
+        // Stages 0..2 store float results (Ur', S); stage 3 forms
+        // Un+1 = (Un + S) + a3 dU in double (main.c:415-420).
         const float kxf = (float)kx, kkf = (float)kk, facf = (float)fac;
         const float kvf[3] = {kxf, (float)ky, (float)kz};
         float2 e[3];
@@ -643,6 +645,9 @@ __device__ __forceinline__ void forward_x_rk(const cx<R> *__restrict__ W, cx<SR>
           Pf.x = (e[0].x * kvf[0] + e[1].x * kvf[1] + e[2].x * kvf[2]) / kkf;
           Pf.y = (e[0].y * kvf[0] + e[1].y * kvf[1] + e[2].y * kvf[2]) / kkf;
         }
+        if constexpr (STAGE == 3) {
+          if (phat) st<double>(phat + l, make_double2((double)Pf.x, (double)Pf.y));
+        }
         const float vkf = nudtf * kkf;
         cx<R> nxt[3];
 #pragma unroll
@@ -652,6 +657,16 @@ __device__ __forceinline__ void forward_x_rk(const cx<R> *__restrict__ W, cx<SR>
           e[f].x -= Pf.x * kvf[f] + vkf * ur.x;
           e[f].y -= Pf.y * kvf[f] + vkf * ur.y;
           const float2 du = e[f];
+          if constexpr (STAGE == 3) {
+            const double2 un = B.un[f][i];
+            const float2 sa = B.sf[f][i];
+            double2 nn = make_double2(un.x + (double)sa.x, un.y + (double)sa.y);
+            nn = make_double2(nn.x + q.a * (double)du.x, nn.y + q.a * (double)du.y);
+            st<double>(uhn + lf, nn);
+            st<float>(inc + lf, cvt<float, double>(nn));
+            nxt[f] = cvt<R, double>(nn);
+            continue;
+          }
           const float2 un = STAGE == 0 ? ur : B.unf[f][i];
           const float2 nr = make_float2(un.x + bf * du.x, un.y + bf * du.y);
           st<SR>(uhr + lf, cvt<SR, float>(nr));
