@@ -218,11 +218,11 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
   static constexpr int T_P = sizeof(R) == 8 ? T_D : (16 * TPP > 1024 ? 1024 / TPP : 16);
   static constexpr int E_X = (sizeof(R) == 4 && E == 8 && N >= 256) ? 16 : E;
   static constexpr int TPP_X = N / E_X;
-  // Epilogue batching of the x pass (EPI_X3: stage 4, kernel STAGE 3): the
+  // Epilogue batching of the x pass, stages 1-3 (EPI_X) and 4 (EPI_X3): the
   // float epilogue of the mixed state leaves room for four positions per
   // load batch with 16-element threads (n=512: 1.94 -> 1.86 ms in stage 1,
   // 2.66 -> 2.64 in stages 2-3, 2.99 -> 2.95 in stage 4; prefetching the
-  // next batch instead costs more).
+  // next batch instead costs more).  The double epilogue keeps the default.
   static constexpr int EPI_X = (sizeof(SR) == 4 && E_X == 16) ? 4 : 0;
   static constexpr int EPI_X3 = EPI_X;
   static constexpr int T_X = sizeof(R) == 8 ? T_D : (16 * TPP_X > 512 ? 512 / TPP_X : 16);
