@@ -418,8 +418,8 @@ __device__ __forceinline__ void zrow(cx<R> *__restrict__ w, UR *__restrict__ usr
 // Output: N = u x omega (main.c:374-376) transformed along z into W 0..2,
 // kz < kmax only (the rest is removed by the dealias mask).
 // (N_x, N_y) are transformed and stored before N_z is transformed, so the
-// two forward transforms are never live together (128 instead of 137
-// VGPRs in float).  Grid n*n/PPW blocks.
+// two forward transforms are never live together (9 VGPRs fewer in float
+// when measured).  Grid n*n/PPW blocks.
 template <int N, typename R, int PPW, int MODE, typename UR, int MINW = 1>
 __global__ __launch_bounds__(PPW *(N / Sched<N>::E), MINW) void k_zfused(cx<R> *__restrict__ W, UR *__restrict__ us,
                                                                     Geo g, R invn3,
