@@ -99,7 +99,9 @@ static int ctx_create(dns_ctx **out, int n, int members, const double *nu, const
   c->g.rs = (long)n * n * n;
   c->g.nk = 0;  // wavenumbers m >= 0 inside the dealias box, |m| < kmax (main.c:237-238)
   while (c->g.nk <= n / 2 && (double)c->g.nk < c->kmax) c->g.nk++;
-  c->g.mW = 6 * c->g.fs;
+  c->g.wps = (long)(n + 1) * c->nfp;  // step partials: one spare row per x-plane
+  c->g.wfs = (long)n * c->g.wps;
+  c->g.mW = 6 * c->g.wfs;
   c->g.mS = 3 * c->g.fs;
   c->g.mR = 3 * c->g.rs;
   if (hipSetDevice(device) != hipSuccess) {
@@ -124,7 +126,7 @@ static int ctx_create(dns_ctx **out, int n, int members, const double *nu, const
   DNS_TRY(alloc(c, &c->uhn, M * 3 * C64));
   DNS_TRY(alloc(c, &c->uhr, M * 3 * impl->sr_size() * c->g.fs));
   DNS_TRY(alloc(c, &c->acc, M * impl->acc_bytes(c->g.fs)));
-  DNS_TRY(alloc(c, &c->W, M * 6 * 2 * impl->r_size() * c->g.fs));
+  DNS_TRY(alloc(c, &c->W, M * 6 * 2 * impl->r_size() * c->g.wfs));
   DNS_TRY(alloc(c, &c->us, M * 3 * impl->r_size() * c->g.rs));
   DNS_TRY(alloc(c, (void **)&c->part, sizeof(double) * 4 * DIAG_BLOCKS));
   DNS_TRY(alloc(c, (void **)&c->dres, sizeof(double) * 4));

@@ -13,8 +13,10 @@
 //   uhr  3U SR      stage state Û_r (the argument of stages 2..4)
 //   acc  3U double  RK4 accumulator (Û1 of main.c)
 //   W    6U R       FFT partials: u,v,w in fields 0..2, omega in 3..5;
-//                   N = u x omega in 0..2 after the fused z pass.  Also
-//                   the scratch of the initial transform and of snapshots.
+//                   N = u x omega in 0..2 after the fused z pass; each
+//                   x-plane holds one spare row (Geo::wps, Geo::wfs).  Also
+//                   the scratch of the initial transform and of snapshots
+//                   (plain layout, fields fs apart).
 //   us   3R R       stale physical velocity (main.c keeps U,V,W from the
 //                   previous stage 4 for stage 1, main.c:344-353)
 //   phat 1U double  stage-4 pressure, allocated on first capture request
@@ -427,20 +429,20 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
         const int sl = DNS_NPASS * rk;
         // pass A of this stage ran fused into the previous stage's E
         if (rk == 0) {
-          DNS_TIMED(c, sl + 1, (k_ypass<N, R, TY, true, 1, PIPE_Y, false, E_Y><<<gy, by, 0, s>>>(W + 3 * g.fs, 3, g, tw, g.mW)));
+          DNS_TIMED(c, sl + 1, (k_ypass<N, R, TY, true, 1, PIPE_Y, false, E_Y, false, true><<<gy, by, 0, s>>>(W + 3 * g.wfs, 3, g, tw, g.mW)));
           DNS_TIMED(c, sl + 2, (k_zfused<N, R, PP, 0, R><<<gz, bz, 0, s>>>(W, (R *)c->us, g, invn3, tw)));
         } else {
           if constexpr (CX_IN_Y)
-            DNS_TIMED(c, sl + 1, (k_ypass<N, R, TY, true, 1, PIPE_YC, false, E_Y, true><<<gy, by, 0, s>>>(W, 6, g, tw, g.mW)));
+            DNS_TIMED(c, sl + 1, (k_ypass<N, R, TY, true, 1, PIPE_YC, false, E_Y, true, true><<<gy, by, 0, s>>>(W, 6, g, tw, g.mW)));
           else
-            DNS_TIMED(c, sl + 1, (k_ypass<N, R, TY, true, 1, PIPE_Y, false, E_Y><<<gy, by, 0, s>>>(W, 6, g, tw, g.mW)));
+            DNS_TIMED(c, sl + 1, (k_ypass<N, R, TY, true, 1, PIPE_Y, false, E_Y, false, true><<<gy, by, 0, s>>>(W, 6, g, tw, g.mW)));
           if (rk < 3)
             DNS_TIMED(c, sl + 2, (k_zfused<N, R, PP, 1, R><<<gz, bz, 0, s>>>(W, (R *)c->us, g, invn3, tw)));
           else
             DNS_TIMED(c, sl + 2, (k_zfused<N, R, PP, 2, R><<<gz, bz, 0, s>>>(W, (R *)c->us, g, invn3, tw)));
         }
         DNS_TIMED(c, sl + 3,
-                  (k_ypass<N, R, TYF, false, 1, PIPE_YF, true, E_Y><<<gy_trunc, byf, 0, s>>>(W, 3, g, tw, g.mW)));
+                  (k_ypass<N, R, TYF, false, 1, PIPE_YF, true, E_Y, false, true><<<gy_trunc, byf, 0, s>>>(W, 3, g, tw, g.mW)));
         // E fused with the next stage's A (the next step's stage 1 after rk 3)
         switch (rk) {
           case 0:

@@ -1,8 +1,9 @@
 // HIP kernels of the RK4 step (gfx950).  All arrays are device-resident.
 //
 // Layouts (i = x, j = y, k = z fastest):
-//   spectral partials (W): complex [n][n][nfp], element (i,j,k) at
-//                     (i*n+j)*nfp+k (x slowest, as main.c:236,358),
+//   spectral partials (W): complex [n][n+1][nfp], element (i,j,k) at
+//                     i*wps + j*nfp + k, wps = (n+1)*nfp (x slowest, as
+//                     main.c:236,358; one spare row per x-plane),
 //                     k < nf = n/2+1 valid, nfp = nf rounded up to 16
 //                     (rows start on 128-byte boundaries);
 //   spectral state (uhn, uhr, acc, phat): complex [n][n][nfp] with y
@@ -41,6 +42,13 @@ struct Geo {
   // same n: blockIdx.z selects the member.  W-type partials, spectral state
   // (uhn/uhr/acc) and stale physical velocity.
   long mW, mS, mR;
+  // The step's partials W use their own strides: an i-plane of n rows plus
+  // one spare row (wps = (n+1)*nfp), so that the x pass's pencils, which
+  // step from plane to plane, do not keep hitting the same HBM channels
+  // (a copy with the unpadded 2.2 MB plane stride of n=512 runs at 4.7 TB/s,
+  // with one spare row 5.1 TB/s); wfs = n*wps per field.  Scratch uses of
+  // the W buffer (transforms of the state, snapshots) keep the fs layout.
+  long wfs, wps;
 };
 
 // Index i of an x or y axis lies inside the dealiased box.
@@ -95,29 +103,32 @@ __global__ __launch_bounds__(256) void k_swap_rows(const T *__restrict__ src, T 
 // In (x, ky, kz) space ky and kz are per-element constants, so this equals
 // the inverse x transform of the spectral curlX.
 template <int N, typename R, int T, bool INV, int MINW = 1, int PIPE = 0, bool TRUNC = false,
-          int E = Sched<N>::E, bool CURLX = false>
+          int E = Sched<N>::E, bool CURLX = false, bool WP = false>
 __global__ __launch_bounds__(T *(N / E), MINW) void k_ypass(cx<R> *__restrict__ data, int nfields, Geo g,
                                                            const cx<R> *__restrict__ tw, long mstride = 0) {
   constexpr int TPP = N / E;
+  // WP: the step's partials (field stride wfs, plane stride wps); otherwise
+  // the plain [n][n][nfp] layout of fields fs apart
+  const long fstr = WP ? g.wfs : g.fs, pstr = WP ? g.wps : (long)N * g.nfp;
   __shared__ cx<R> lds[T * padlen<N>()];
   __shared__ cx<R> twl[tw_size<N>()];
   tw = stage_twiddles<N, R>(twl, tw);
   const int p = threadIdx.x % T, t = threadIdx.x / T;
   const int k = blockIdx.x * T + p;
   const bool ok = k < (TRUNC ? g.nk : N / 2 + 1);
-  cx<R> *base = data + blockIdx.z * mstride + (long)blockIdx.y * N * g.nfp + k;
+  cx<R> *base = data + blockIdx.z * mstride + (long)blockIdx.y * pstr + k;
   // loaded fields: all, or 0,1,2,4,5 with CURLX
   const int nload = CURLX ? 5 : nfields;
   auto fld = [](int i) { return CURLX && i >= 3 ? i + 1 : i; };
   auto load = [&](cx<R>(&v)[E], int f) {
-    const cx<R> *d = base + f * g.fs;
+    const cx<R> *d = base + f * fstr;
 #pragma unroll
     for (int c = 0; c < E; c++) v[c] = ok ? d[(long)(t + TPP * c) * g.nfp] : zero<R>();
   };
   auto fft_store = [&](cx<R>(&v)[E], int f) {
     fft_pencil<N, R, INV>(v, t, lds, TileSlot<N, T>{p}, tw);
     if (ok) {
-      cx<R> *d = base + f * g.fs;
+      cx<R> *d = base + f * fstr;
 #pragma unroll
       for (int c = 0; c < E; c++)
         if (!TRUNC || in_box(t + TPP * c, N, g.nk)) st<R>(d + (long)(t + TPP * c) * g.nfp, v[c]);
@@ -212,13 +223,13 @@ __device__ __forceinline__ void curl_inverse_x(cx<R> (&U)[E], cx<R> (&V)[E], cx<
                                                cx<R> *__restrict__ W, Geo g, int t, int p, int j, int k, bool ok,
                                                cx<R> *lds, const cx<R> *__restrict__ tw) {
   constexpr int TPP = N / E;
-  const long off = (long)j * g.nfp + k, istr = (long)N * g.nfp;
+  const long off = (long)j * g.nfp + k, istr = g.wps;  // step partials
   const R ky = (R)wave<N>(j), kz = (R)k;
   const TileSlot<N, T> slot{p};
   auto store = [&](int f, cx<R>(&v)[E]) {
     if (ok) {
 #pragma unroll
-      for (int c = 0; c < E; c++) st<R>(W + f * g.fs + off + (t + TPP * c) * istr, v[c]);
+      for (int c = 0; c < E; c++) st<R>(W + f * g.wfs + off + (t + TPP * c) * istr, v[c]);
     }
   };
   auto irot = [](R a, R b) { return mk<R>(-b, a); };  // I*(a + i b)
@@ -365,7 +376,7 @@ __device__ __forceinline__ void zrow(cx<R> *__restrict__ w, UR *__restrict__ usr
       v[c] = (R)usr[g.rs + l];
       ww[c] = (R)usr[2 * g.rs + l];
     }
-    herm_pair<N, R>(z1, w + 3 * g.fs, w + 4 * g.fs, t);
+    herm_pair<N, R>(z1, w + 3 * g.wfs, w + 4 * g.wfs, t);
     fft_pencil<N, R, true>(z1, t, lds, slot, tw);
 #pragma unroll
     for (int c = 0; c < E; c++) {
@@ -377,9 +388,9 @@ __device__ __forceinline__ void zrow(cx<R> *__restrict__ w, UR *__restrict__ usr
     cx<R> z1[E], z2[E], z3[E];
     herm_combine<N, R>(z1, h, t);
     fft_pencil<N, R, true>(z1, t, lds, slot, tw);
-    herm_pair<N, R>(z2, w + 2 * g.fs, w + 3 * g.fs, t);
+    herm_pair<N, R>(z2, w + 2 * g.wfs, w + 3 * g.wfs, t);
     fft_pencil<N, R, true>(z2, t, lds, slot, tw);
-    herm_pair<N, R>(z3, w + 4 * g.fs, w + 5 * g.fs, t);
+    herm_pair<N, R>(z3, w + 4 * g.wfs, w + 5 * g.wfs, t);
     fft_pencil<N, R, true>(z3, t, lds, slot, tw);
 #pragma unroll
     for (int c = 0; c < E; c++) {
@@ -407,7 +418,7 @@ __device__ __forceinline__ void zrow(cx<R> *__restrict__ w, UR *__restrict__ usr
     nz[c] = u[c] * oy[c] - v[c] * ox[c];
   }
   fft_pencil<N, R, false>(f1, t, lds, slot, tw);
-  split_store<N, R>(f1, w, w + g.fs, t, lds, slot, g.nk);
+  split_store<N, R>(f1, w, w + g.wfs, t, lds, slot, g.nk);
 }
 
 // C: fused z pass of one RK stage, one row per wave.
@@ -429,18 +440,18 @@ __global__ __launch_bounds__(PPW *(N / Sched<N>::E), MINW) void k_zfused(cx<R> *
   const int p = threadIdx.x / TPP, t = threadIdx.x % TPP;
   const long row = (long)blockIdx.x * PPW + p;
   us += blockIdx.z * g.mR;
-  cx<R> *w = W + blockIdx.z * g.mW + row * g.nfp;
+  cx<R> *w = W + blockIdx.z * g.mW + (row / N) * g.wps + (row % N) * g.nfp;
   const PencilSlot<N> slot{p * padlen<N>()};
   R oz0[E], nz[E];
   HermRaw<N, R> h;
   if constexpr (MODE == 0) {
     cx<R> z[E];
-    herm_pair<N, R>(z, w + 5 * g.fs, nullptr, t);
+    herm_pair<N, R>(z, w + 5 * g.wfs, nullptr, t);
     fft_pencil<N, R, true>(z, t, lds, slot, tw);
 #pragma unroll
     for (int c = 0; c < E; c++) oz0[c] = z[c].x * invn3;
   } else {
-    herm_load<N, R>(h, w, w + g.fs, t);
+    herm_load<N, R>(h, w, w + g.wfs, t);
   }
   zrow<N, R, MODE, UR>(w, us + row * N, g, invn3, tw, t, lds, slot, oz0, nz, h);
   // N_z is the transform of a real pencil: its first half is the r2c output.
@@ -451,7 +462,7 @@ __global__ __launch_bounds__(PPW *(N / Sched<N>::E), MINW) void k_zfused(cx<R> *
 #pragma unroll
   for (int c = 0; c <= E / 2; c++) {
     const int pos = t + TPP * c;
-    if (pos <= H && pos < g.nk) st<R>(w + 2 * g.fs + pos, f2[c]);
+    if (pos <= H && pos < g.nk) st<R>(w + 2 * g.wfs + pos, f2[c]);
   }
 }
 
@@ -537,7 +548,7 @@ __device__ __forceinline__ void forward_x_rk(const cx<R> *__restrict__ W, cx<SR>
                                              cx<R> *lds, cx<R> (&U)[E], cx<R> (&V)[E], cx<R> (&Wv)[E]) {
 #pragma clang fp contract(off)
   constexpr int TPP = N / E;
-  const long off = (long)j * g.nfp + k, istr = (long)N * g.nfp;
+  const long off = (long)j * g.nfp + k, istr = g.wps;  // step partials
   // state element (i, j, k) at soff + i * sstr (state layout)
   const long soff = (long)j * N * g.nfp + k, sstr = g.nfp;
   const TileSlot<N, T> slot{p};
@@ -566,7 +577,7 @@ __device__ __forceinline__ void forward_x_rk(const cx<R> *__restrict__ W, cx<SR>
 #pragma unroll
     for (int f = 0; f < 3; f++) {
 #pragma unroll
-      for (int c = 0; c < E; c++) F[f][c] = okn ? W[f * g.fs + off + (t + TPP * c) * istr] : zero<R>();
+      for (int c = 0; c < E; c++) F[f][c] = okn ? W[f * g.wfs + off + (t + TPP * c) * istr] : zero<R>();
     }
 #pragma unroll
     for (int f = 0; f < 3; f++) fft_pencil<N, R, false>(F[f], t, lds, slot, tw);

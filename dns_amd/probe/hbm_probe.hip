@@ -4,6 +4,9 @@
 //
 //   hbm_probe            table of read-only / write-only / copy kernels
 //   hbm_probe --calib    one copy of 1 GiB (reads 1 GiB, writes 1 GiB)
+//   hbm_probe --planes   pencil-tile copies in the layout of the step's
+//                        partials at n=512 (x pencils with and without a
+//                        spare row per x-plane, y pencils)
 #include <hip/hip_runtime.h>
 #include <cstdio>
 #include <cstdlib>
@@ -43,6 +46,28 @@ __global__ void k_fill(v2d *__restrict__ b, long n) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += stride) b[i] = v2d{1.0, 2.0};
 }
 
+// Copy of 8-column tiles (k) along the pencils of one axis of a complex
+// [512][rows][272] layout, 8 elements per thread as in the passes: XPEN
+// walks the slowest index (plane stride ps), otherwise the middle one (row
+// stride 272).  Grid (34, 512), 512 threads.
+template <bool XPEN>
+__global__ __launch_bounds__(512) void k_tile_copy(const v2d *__restrict__ a, v2d *__restrict__ b, long ps) {
+  constexpr int N = 512, NFP = 272, T = 8, E = 8, TPP = N / E;
+  const int p = threadIdx.x % T, t = threadIdx.x / T;
+  const long k = (long)blockIdx.x * T + p, fixed = blockIdx.y;
+  v2d v[E];
+#pragma unroll
+  for (int c = 0; c < E; c++) {
+    const long m = t + TPP * c;
+    v[c] = a[XPEN ? m * ps + fixed * NFP + k : fixed * ps + m * NFP + k];
+  }
+#pragma unroll
+  for (int c = 0; c < E; c++) {
+    const long m = t + TPP * c;
+    __builtin_nontemporal_store(v[c], b + (XPEN ? m * ps + fixed * NFP + k : fixed * ps + m * NFP + k));
+  }
+}
+
 static hipStream_t st;
 
 template <class F> static void timeit(const char *name, double bytes, F f) {
@@ -67,6 +92,7 @@ template <class F> static void timeit(const char *name, double bytes, F f) {
 
 int main(int argc, char **argv) {
   const bool calib = argc > 1 && strcmp(argv[1], "--calib") == 0;
+  const bool planes = argc > 1 && strcmp(argv[1], "--planes") == 0;
   CK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
   const long n = calib ? (1L << 26) : (1L << 28);  // 1 GiB or 4 GiB of 16-byte elements
   v2d *a, *b;
@@ -81,6 +107,15 @@ int main(int argc, char **argv) {
     CK(hipGetLastError());
     CK(hipStreamSynchronize(st));
     printf("calibration copy: read %ld bytes, wrote %ld bytes\n", n * 16, n * 16);
+    return 0;
+  }
+  if (planes) {  // largest index (511*513+511)*272+271 < n
+    printf("| tile copy, [512][rows][272] complex double | ms | GB/s | %% of 8 TB/s |\n|---|---|---|---|\n");
+    const double tb = 2.0 * 16 * 512 * 512 * 272;
+    const dim3 grid(272 / 8, 512);
+    timeit("x pencils, 512 rows a plane", tb, [&] { k_tile_copy<true><<<grid, 512, 0, st>>>(a, b, 512L * 272); });
+    timeit("x pencils, 513 rows a plane", tb, [&] { k_tile_copy<true><<<grid, 512, 0, st>>>(a, b, 513L * 272); });
+    timeit("y pencils", tb, [&] { k_tile_copy<false><<<grid, 512, 0, st>>>(a, b, 512L * 272); });
     return 0;
   }
   printf("| kernel (4 GiB arrays) | ms | GB/s | %% of 8 TB/s |\n|---|---|---|---|\n");
