@@ -5,9 +5,9 @@ Every grid size compiles its own launch shapes: 16-column float tiles with
 radix-3 schedules of 192, 384 and 768, the two-wave z pencils of n=1024.
 The small-n parity tests do not run these.  Here: FP64 steps against the
 CPU oracle up to n=512, mixed against FP64 on the same grids, and the
-Beltrami closed form at n=768 (the oracle would take minutes there).  The
-n=1024 instance is checked by `bench.py --n 1024`, whose FP64 and mixed
-runs (different curl paths at that size) agree in energy to ~4e-13.
+Beltrami closed form at n=768 and n=1024 (the oracle would take minutes
+there; at n=1024 `bench.py --n 1024` also compares FP64 and mixed, whose
+curl paths differ at that size: energy within ~4e-13).
 
 Inputs are sums of separable Fourier modes (cheap to build at 512^3),
 including modes beyond the 2/3 cutoff and on the Nyquist planes."""
@@ -112,6 +112,25 @@ def test_beltrami_n768(precision, tol):
         s.load_velocity(beltrami_broadcast(n, k))
         e0, z0 = s.diagnostics()
         assert abs(z0 - k * k * e0) <= 1e-13 * z0   # a single shell |k| = k
+        s.step(m)
+        e, z = s.diagnostics()
+        f = rk4_amplification(nu * dt * k * k) ** (2 * m)
+        assert abs(e - e0 * f) / (e0 * f) < tol and abs(z - z0 * f) / (z0 * f) < tol
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("mixed", 1e-6)])
+def test_beltrami_n1024(precision, tol):
+    """The n=1024 instances (FP64: 16-element y passes, 4-column x tiles that
+    also transform omega_x, about 160 GB of device memory; both: z pencils
+    over two waves): energy and enstrophy decay by R^2 per step."""
+    n, nu, dt, k, m = 1024, 0.002, 0.01, 9, 2
+    print(f"beltrami {precision} n={n}", flush=True)
+    with dns_amd.Solver(n, nu, dt, precision) as s:
+        u = beltrami_broadcast(n, k)
+        s.load_velocity(u)
+        del u
+        e0, z0 = s.diagnostics()
+        assert abs(z0 - k * k * e0) <= 1e-13 * z0
         s.step(m)
         e, z = s.diagnostics()
         f = rk4_amplification(nu * dt * k * k) ** (2 * m)
