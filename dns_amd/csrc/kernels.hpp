@@ -110,6 +110,10 @@ __global__ __launch_bounds__(T *(N / E), MINW) void k_ypass(cx<R> *__restrict__ 
   // WP: the step's partials (field stride wfs, plane stride wps); otherwise
   // the plain [n][n][nfp] layout of fields fs apart
   const long fstr = WP ? g.wfs : g.fs, pstr = WP ? g.wps : (long)N * g.nfp;
+  // non-temporal loads (each element is read once), except in the float
+  // forward pass, which they slow down (n=512: 1.47 -> 1.60 ms/step, while
+  // the double one gains 2.84 -> 2.75 and the inverse passes 4-5%)
+  constexpr bool NTL = !TRUNC || sizeof(R) == 8;
   __shared__ cx<R> lds[T * padlen<N>()];
   __shared__ cx<R> twl[tw_size<N>()];
   tw = stage_twiddles<N, R>(twl, tw);
@@ -123,7 +127,10 @@ __global__ __launch_bounds__(T *(N / E), MINW) void k_ypass(cx<R> *__restrict__ 
   auto load = [&](cx<R>(&v)[E], int f) {
     const cx<R> *d = base + f * fstr;
 #pragma unroll
-    for (int c = 0; c < E; c++) v[c] = ok ? d[(long)(t + TPP * c) * g.nfp] : zero<R>();
+    for (int c = 0; c < E; c++) {
+      const cx<R> *q = d + (long)(t + TPP * c) * g.nfp;
+      v[c] = ok ? (NTL ? ldnt(q) : *q) : zero<R>();
+    }
   };
   auto fft_store = [&](cx<R>(&v)[E], int f) {
     fft_pencil<N, R, INV>(v, t, lds, TileSlot<N, T>{p}, tw);
@@ -284,9 +291,9 @@ __global__ __launch_bounds__(T *(N / E), MINW) void k_xinv_curl(const cx<SI> *__
 #pragma unroll
   for (int c = 0; c < E; c++) {
     const long l = soff + (t + TPP * c) * g.nfp;
-    U[c] = ok ? cvt<R, SI>(uh[l]) : zero<R>();
-    V[c] = ok ? cvt<R, SI>(uh[g.fs + l]) : zero<R>();
-    Wv[c] = ok ? cvt<R, SI>(uh[2 * g.fs + l]) : zero<R>();
+    U[c] = ok ? cvt<R, SI>(ldnt(uh + l)) : zero<R>();
+    V[c] = ok ? cvt<R, SI>(ldnt(uh + g.fs + l)) : zero<R>();
+    Wv[c] = ok ? cvt<R, SI>(ldnt(uh + 2 * g.fs + l)) : zero<R>();
   }
   curl_inverse_x<N, R, T, WITH_U, E>(U, V, Wv, W, g, t, p, j, k, ok, lds, tw);
 }
@@ -577,7 +584,7 @@ __device__ __forceinline__ void forward_x_rk(const cx<R> *__restrict__ W, cx<SR>
 #pragma unroll
     for (int f = 0; f < 3; f++) {
 #pragma unroll
-      for (int c = 0; c < E; c++) F[f][c] = okn ? W[f * g.wfs + off + (t + TPP * c) * istr] : zero<R>();
+      for (int c = 0; c < E; c++) F[f][c] = okn ? ldnt(W + f * g.wfs + off + (t + TPP * c) * istr) : zero<R>();
     }
 #pragma unroll
     for (int f = 0; f < 3; f++) fft_pencil<N, R, false>(F[f], t, lds, slot, tw);
@@ -615,16 +622,16 @@ __device__ __forceinline__ void forward_x_rk(const cx<R> *__restrict__ W, cx<SR>
 #pragma unroll
       for (int f = 0; f < 3; f++) {
         const long lf = f * g.fs + soff + (t + TPP * (c0 + i)) * sstr;
-        if constexpr (STAGE == 0 && MIXED_STATE) B.ur[f][i] = inc[lf];
-        else if constexpr (STAGE == 0) B.ur[f][i] = uhn[lf];
-        else B.ur[f][i] = uhr[lf];
+        if constexpr (STAGE == 0 && MIXED_STATE) B.ur[f][i] = ldnt(inc + lf);
+        else if constexpr (STAGE == 0) B.ur[f][i] = ldnt(uhn + lf);
+        else B.ur[f][i] = ldnt(uhr + lf);
         if constexpr (MIXED_STATE) {
-          if constexpr (STAGE == 1 || STAGE == 2) B.unf[f][i] = inc[lf];
-          if constexpr (STAGE == 3) B.un[f][i] = uhn[lf];
-          if constexpr (STAGE > 0) B.sf[f][i] = inc[3 * g.fs + lf];
+          if constexpr (STAGE == 1 || STAGE == 2) B.unf[f][i] = ldnt(inc + lf);
+          if constexpr (STAGE == 3) B.un[f][i] = ldnt(uhn + lf);
+          if constexpr (STAGE > 0) B.sf[f][i] = ldnt(inc + 3 * g.fs + lf);
         } else {
-          if constexpr (STAGE == 1 || STAGE == 2) B.un[f][i] = uhn[lf];
-          if constexpr (STAGE > 0) B.ac[f][i] = acc[lf];
+          if constexpr (STAGE == 1 || STAGE == 2) B.un[f][i] = ldnt(uhn + lf);
+          if constexpr (STAGE > 0) B.ac[f][i] = ldnt(acc + lf);
         }
       }
     }
