@@ -129,7 +129,7 @@ __global__ __launch_bounds__(T *(N / E), MINW) void k_ypass(cx<R> *__restrict__ 
 #pragma unroll
     for (int c = 0; c < E; c++) {
       const cx<R> *q = d + (long)(t + TPP * c) * g.nfp;
-      v[c] = ok ? (NTL ? ldnt(q) : *q) : zero<R>();
+      v[c] = ok ? (NTL ? ld_row<T>(q) : *q) : zero<R>();
     }
   };
   auto fft_store = [&](cx<R>(&v)[E], int f) {
@@ -291,9 +291,9 @@ __global__ __launch_bounds__(T *(N / E), MINW) void k_xinv_curl(const cx<SI> *__
 #pragma unroll
   for (int c = 0; c < E; c++) {
     const long l = soff + (t + TPP * c) * g.nfp;
-    U[c] = ok ? cvt<R, SI>(ldnt(uh + l)) : zero<R>();
-    V[c] = ok ? cvt<R, SI>(ldnt(uh + g.fs + l)) : zero<R>();
-    Wv[c] = ok ? cvt<R, SI>(ldnt(uh + 2 * g.fs + l)) : zero<R>();
+    U[c] = ok ? cvt<R, SI>(ld_row<T>(uh + l)) : zero<R>();
+    V[c] = ok ? cvt<R, SI>(ld_row<T>(uh + g.fs + l)) : zero<R>();
+    Wv[c] = ok ? cvt<R, SI>(ld_row<T>(uh + 2 * g.fs + l)) : zero<R>();
   }
   curl_inverse_x<N, R, T, WITH_U, E>(U, V, Wv, W, g, t, p, j, k, ok, lds, tw);
 }
@@ -584,7 +584,7 @@ __device__ __forceinline__ void forward_x_rk(const cx<R> *__restrict__ W, cx<SR>
 #pragma unroll
     for (int f = 0; f < 3; f++) {
 #pragma unroll
-      for (int c = 0; c < E; c++) F[f][c] = okn ? ldnt(W + f * g.wfs + off + (t + TPP * c) * istr) : zero<R>();
+      for (int c = 0; c < E; c++) F[f][c] = okn ? ld_row<T>(W + f * g.wfs + off + (t + TPP * c) * istr) : zero<R>();
     }
 #pragma unroll
     for (int f = 0; f < 3; f++) fft_pencil<N, R, false>(F[f], t, lds, slot, tw);
@@ -622,16 +622,16 @@ __device__ __forceinline__ void forward_x_rk(const cx<R> *__restrict__ W, cx<SR>
 #pragma unroll
       for (int f = 0; f < 3; f++) {
         const long lf = f * g.fs + soff + (t + TPP * (c0 + i)) * sstr;
-        if constexpr (STAGE == 0 && MIXED_STATE) B.ur[f][i] = ldnt(inc + lf);
-        else if constexpr (STAGE == 0) B.ur[f][i] = ldnt(uhn + lf);
-        else B.ur[f][i] = ldnt(uhr + lf);
+        if constexpr (STAGE == 0 && MIXED_STATE) B.ur[f][i] = ld_row<T>(inc + lf);
+        else if constexpr (STAGE == 0) B.ur[f][i] = ld_row<T>(uhn + lf);
+        else B.ur[f][i] = ld_row<T>(uhr + lf);
         if constexpr (MIXED_STATE) {
-          if constexpr (STAGE == 1 || STAGE == 2) B.unf[f][i] = ldnt(inc + lf);
-          if constexpr (STAGE == 3) B.un[f][i] = ldnt(uhn + lf);
-          if constexpr (STAGE > 0) B.sf[f][i] = ldnt(inc + 3 * g.fs + lf);
+          if constexpr (STAGE == 1 || STAGE == 2) B.unf[f][i] = ld_row<T>(inc + lf);
+          if constexpr (STAGE == 3) B.un[f][i] = ld_row<T>(uhn + lf);
+          if constexpr (STAGE > 0) B.sf[f][i] = ld_row<T>(inc + 3 * g.fs + lf);
         } else {
-          if constexpr (STAGE == 1 || STAGE == 2) B.un[f][i] = ldnt(uhn + lf);
-          if constexpr (STAGE > 0) B.ac[f][i] = ldnt(acc + lf);
+          if constexpr (STAGE == 1 || STAGE == 2) B.un[f][i] = ld_row<T>(uhn + lf);
+          if constexpr (STAGE > 0) B.ac[f][i] = ld_row<T>(acc + lf);
         }
       }
     }
