@@ -210,3 +210,40 @@ def test_verbose_lines(tmp_path):
     head, _, rest = lines[2].partition(" steps in ")
     assert head == "dns: 20" and rest.endswith(" steps/s)"), lines[2]
     assert q.stderr == ""
+
+
+def test_unsupported_grid_size(tmp_path):
+    """Grids are n = 2^a 3^b, 16..1024: a 100^3 file is read (its size
+    passes main.c:168's check) and then refused by the library."""
+    p = tmp_path / "n100.raw"
+    np.zeros(4 * 100 ** 3).tofile(p)
+    print("unsupported grid n=100: dns", flush=True)
+    r = _run_any(DNS_CLI, ["-i", str(p), "-n", "0.01", "-s", "0.01", "-t", "0.1"], tmp_path)
+    assert r.returncode == 1
+    assert r.stderr == "dns: error: invalid argument (unsupported grid size n=100)\n"
+    assert r.stdout == ""
+
+
+def test_mixed_dump_files_close_to_fp64(tmp_path):
+    """-d under DNS_PRECISION=mixed: the same files as FP64 (names, XDMF
+    bytes), raw velocity within 1e-5 and pressure within 1e-4 of the FP64
+    dump's max (the pressure comes from the float projection)."""
+    n = 32
+    from conftest import random_field
+    raw = tmp_path / "in.raw"
+    write_raw(raw, random_field(n, 19))
+    gdir, mdir = tmp_path / "g", tmp_path / "m"
+    gdir.mkdir()
+    mdir.mkdir()
+    args = ["-d", "-i", str(raw), "-n", "0.02", "-s", "0.01", "-t", "0.105"]
+    print("dumps: dns fp64 and mixed", flush=True)
+    _run(DNS_CLI, args, gdir)
+    _run(DNS_CLI, args, mdir, env={"DNS_PRECISION": "mixed"})
+    assert sorted(os.listdir(gdir)) == sorted(os.listdir(mdir))
+    for f in ("a.00000000.xdmf2", "a.00000001.xdmf2"):
+        assert (gdir / f).read_bytes() == (mdir / f).read_bytes()
+    for f in ("00000000.raw", "00000010.raw"):
+        a = np.fromfile(gdir / f).reshape(4, n, n, n)
+        b = np.fromfile(mdir / f).reshape(4, n, n, n)
+        assert np.max(np.abs(b[:3] - a[:3])) <= 1e-5 * np.max(np.abs(a[:3])), f
+        assert np.max(np.abs(b[3] - a[3])) <= 1e-4 * max(np.max(np.abs(a[3])), 1e-300) or not np.any(a[3]), f

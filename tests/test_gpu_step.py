@@ -200,3 +200,9 @@ def test_divergence_decays_by_rk4_polynomial(precision, tol):
         want = d0 * rk4_amplification(nu * dt * (kx ** 2 + ky ** 2 + kz ** 2)) ** m
         got = divergence_hat(s.get_spectral())
         assert np.max(np.abs(got - want)) / np.max(np.abs(d0)) < tol
+
+
+def test_unsupported_grid_size_raises():
+    with pytest.raises(dns_amd.DnsError) as e:
+        dns_amd.Solver(100, 0.01, 0.01)
+    assert e.value.code == -1 and "unsupported grid size n=100" in str(e.value)
