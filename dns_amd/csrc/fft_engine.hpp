@@ -58,8 +58,9 @@ template <typename R> __device__ __forceinline__ void st(cx<R> *p, cx<R> v) {
 }
 // Non-temporal load of a complex element that its pass reads once: the x
 // pass with these loads runs 3.7% (FP64) and 5% (float) faster at n=512,
-// the inverse y pass 4%.  Not for the z pass, whose Hermitian loads read
-// each element twice (there they cost 7%).
+// the inverse y pass 4%, the z pass 3-5% (its Hermitian loads fetch each
+// element once and pass the mirrored half between lanes; loading both
+// halves non-temporally instead costs 7%).
 template <typename T> __device__ __forceinline__ T ldnt(const T *p) {
   typedef decltype(p->x) E;
   typedef E v2 __attribute__((ext_vector_type(2)));

@@ -309,15 +309,61 @@ __global__ __launch_bounds__(T *(N / E), MINW) void k_xinv_curl(const cx<SI> *__
 template <int N, typename R> struct HermRaw {
   cx<R> x[Sched<N>::E], y[Sched<N>::E];
 };
+// Value of v in lane `src` of this wave (ds_bpermute; every lane active).
+template <typename R> __device__ __forceinline__ cx<R> lane_get(cx<R> v, int src) {
+  if constexpr (sizeof(R) == 4) {
+    return mk<R>(__int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(v.x))),
+                 __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(v.y))));
+  } else {
+    auto get = [&](double d) {
+      const long long b = __double_as_longlong(d);
+      const int lo = __builtin_amdgcn_ds_bpermute(src << 2, (int)(b & 0xffffffffLL));
+      const int hi = __builtin_amdgcn_ds_bpermute(src << 2, (int)(b >> 32));
+      return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+    };
+    return mk<R>(get(v.x), get(v.y));
+  }
+}
 template <int N, typename R>
 __device__ __forceinline__ void herm_load(HermRaw<N, R> &h, const cx<R> *X, const cx<R> *Y, int t) {
   constexpr int E = Sched<N>::E, TPP = N / E, H = N / 2;
+  if constexpr (TPP <= 64) {
+    // Each stored element is loaded once (non-temporal): positions pos <= H
+    // directly; pos > H needs k = N - pos = TPP (E - c) - t, which thread
+    // TPP - t of the same pencil (same wave) holds at index E - 1 - c (for
+    // t = 0: its own index E - c, or the Nyquist element at c = E/2).
 #pragma unroll
-  for (int c = 0; c < E; c++) {
-    const int pos = t + TPP * c;
-    const int k = pos <= H ? pos : N - pos;
-    h.x[c] = X[k];
-    h.y[c] = Y ? Y[k] : zero<R>();
+    for (int c = 0; c < E / 2; c++) {
+      h.x[c] = ldnt(X + t + TPP * c);
+      h.y[c] = Y ? ldnt(Y + t + TPP * c) : zero<R>();
+    }
+    cx<R> nx = zero<R>(), ny = zero<R>();
+    if (t == 0) {
+      nx = ldnt(X + H);
+      if (Y) ny = ldnt(Y + H);
+    }
+    const int lane = (int)(threadIdx.x & 63);
+    const int src = lane - t + (TPP - t) % TPP;
+#pragma unroll
+    for (int c = E / 2; c < E; c++) {
+      const cx<R> mx = lane_get<R>(h.x[E - 1 - c], src);
+      const cx<R> my = lane_get<R>(h.y[E - 1 - c], src);
+      if (t == 0) {
+        h.x[c] = c == E / 2 ? nx : h.x[E - c];
+        h.y[c] = c == E / 2 ? ny : h.y[E - c];
+      } else {
+        h.x[c] = mx;
+        h.y[c] = my;
+      }
+    }
+  } else {  // a pencil over several waves: each thread loads its mirrored positions too
+#pragma unroll
+    for (int c = 0; c < E; c++) {
+      const int pos = t + TPP * c;
+      const int k = pos <= H ? pos : N - pos;
+      h.x[c] = X[k];
+      h.y[c] = Y ? Y[k] : zero<R>();
+    }
   }
 }
 template <int N, typename R>
