@@ -44,9 +44,11 @@ enum {
 
 /* Arithmetic of the step.
  * DNS_FP64 : everything in double (the reference's arithmetic).
- * DNS_MIXED: FFTs, curl and u x omega in float with twiddles generated in
- *            double; energy/enstrophy reductions, the RK4 accumulator and
- *            the step-start spectral state in double. */
+ * DNS_MIXED: FFTs, curl, u x omega, projection and viscous term in float
+ *            with twiddles generated in double; the stage increments are
+ *            summed in float (S), and the step-start spectral state U_n,
+ *            the new state U_n+1 = (U_n + S) + a_4 dU_4, and the
+ *            energy/enstrophy reductions are double. */
 enum dns_precision { DNS_FP64 = 0, DNS_MIXED = 1 };
 
 typedef struct dns_ctx dns_ctx;
