@@ -76,8 +76,22 @@ def gpu_run(dns_amd, n, nu, dt, precision, u, warmup, steps):
             "passes": per_pass}
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(n, nu, dt, u, steps):
-    """Times the CPU oracle's RK4 step (oracle/_build/liboracle.so)."""
+    """Times the CPU oracle's RK4 step (oracle/_build/liboracle.so) on all
+    host cores, threads bound close (OMP_PROC_BIND, read by the OpenMP
+    runtime when the library loads)."""
+    os.environ.setdefault("OMP_PROC_BIND", "close")
     path = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
     L = ctypes.CDLL(path)
     L.orc_create.restype = ctypes.c_void_p
@@ -92,20 +106,11 @@ def cpu_baseline(n, nu, dt, u, steps):
     wall = time.perf_counter() - t0
     L.orc_destroy(p)
     return {"value": steps / wall, "unit": "steps/s", "cores": int(L.orc_threads()), "kind": "port",
+            "cpu_model": cpu_model(), "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+            "host_cpus": os.cpu_count(),
             "sample": f"{steps} RK4 step(s) of the same n={n} Taylor-Green workload, FP64, "
                       f"{wall:.1f} s wall, OpenMP C restatement of main.c with its own FFT "
                       f"(FFTW is not installed)"}
-
-
-def iso_field(n, seed):
-    """Seeded isotropic field (E(k) ~ k^4 exp(-2 (k/4)^2), energy 0.5) from dns-ic."""
-    import subprocess
-    import tempfile
-    with tempfile.TemporaryDirectory() as d:
-        path = os.path.join(d, "iso.raw")
-        subprocess.run([os.path.join(ROOT, "dns_amd", "bin", "dns-ic"), "-t", "iso", "-N", str(n), "-S", str(seed),
-                        "-o", path], check=True, capture_output=True)
-        return np.fromfile(path).reshape(4, n, n, n)[:3].copy()
 
 
 def ensemble(a):

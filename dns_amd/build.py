@@ -73,11 +73,20 @@ def build_tools(verbose=False):
     ic = os.path.join(BIN, "dns-ic")
     _run([CC, *cflags, os.path.join(PKG, "cli", "dns_ic.c"), f"-L{PKG}", "-ldns_hip",
           "-Wl,-rpath,$ORIGIN/..", "-lm", "-o", ic])
+    if verbose:
+        print("built", dns, ic)
+    return dns
+
+
+def build_probe(verbose=False):
+    """The HBM calibration probe (copy kernels of known size that scale the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE counters); built only on request."""
+    os.makedirs(BIN, exist_ok=True)
     probe = os.path.join(BIN, "hbm_probe")
     _run([HIPCC, *HIP_FLAGS, os.path.join(PKG, "probe", "hbm_probe.hip"), "-o", probe])
     if verbose:
-        print("built", dns, ic, probe)
-    return dns
+        print("built", probe)
+    return probe
 
 
 def build_oracle(verbose=False):
@@ -92,12 +101,15 @@ def build_oracle(verbose=False):
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--oracle", action="store_true", help="also build the CPU oracle")
+    ap.add_argument("--probe", action="store_true", help="also build the HBM calibration probe")
     ap.add_argument("-j", type=int, default=None)
     a = ap.parse_args(argv)
     build_library(a.j, verbose=True)
     build_tools(verbose=True)
     if a.oracle:
         build_oracle(verbose=True)
+    if a.probe:
+        build_probe(verbose=True)
 
 
 if __name__ == "__main__":

@@ -12,6 +12,7 @@ import numpy as np
 import pytest
 
 from conftest import DNS_CLI, DNS_IC, ORACLE_CLI, parse_lines, write_raw, tgv_field
+from dump_check import check_dump_dir
 
 pytestmark = pytest.mark.gpu
 
@@ -75,6 +76,8 @@ def test_dump_files_match_oracle(tmp_path):
         ["00000000.raw", "00000010.raw", "a.00000000.xdmf2", "a.00000001.xdmf2"]
     for f in ("a.00000000.xdmf2", "a.00000001.xdmf2"):
         assert (gdir / f).read_bytes() == (odir / f).read_bytes()
+    # the sidecars and file set against main.c:269-330 directly
+    check_dump_dir(str(gdir), n, 0.01, 10, u, tol=1e-14)
     for f in ("00000000.raw", "00000010.raw"):
         a = np.fromfile(gdir / f).reshape(4, n, n, n)
         b = np.fromfile(odir / f).reshape(4, n, n, n)
