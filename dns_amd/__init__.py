@@ -53,6 +53,8 @@ def lib():
         "dns_set_pressure_capture": (I, [P, I]),
         "dns_diagnostics": (I, [P, D, D]),
         "dns_snapshot": (I, [P, P]),
+        "dns_snapshot_begin": (I, [P]),
+        "dns_snapshot_wait": (I, [P, ctypes.POINTER(ctypes.POINTER(ctypes.c_double))]),
         "dns_get_spectral": (I, [P, P]),
         "dns_set_spectral": (I, [P, P]),
         "dns_get_stale_velocity": (I, [P, P]),
@@ -190,6 +192,16 @@ class Solver:
         out = np.empty((4, self.n, self.n, self.n), dtype=np.float64)
         _check(lib().dns_snapshot(self._ctx, _ptr(out)), "dns_snapshot")
         return out
+
+    def snapshot_begin(self):
+        _check(lib().dns_snapshot_begin(self._ctx), "dns_snapshot_begin")
+
+    def snapshot_wait(self):
+        """A copy of the pinned buffer of the last snapshot_begin."""
+        p = ctypes.POINTER(ctypes.c_double)()
+        _check(lib().dns_snapshot_wait(self._ctx, ctypes.byref(p)), "dns_snapshot_wait")
+        n = self.n
+        return np.ctypeslib.as_array(p, shape=(4, n, n, n)).copy()
 
     def get_spectral(self):
         out = np.empty((3, self.n, self.n, self.nf), dtype=np.complex128)

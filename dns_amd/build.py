@@ -39,22 +39,41 @@ def _run(cmd):
     return out
 
 
-def sources():
-    return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
+def grid_sizes():
+    """Grid sizes listed in csrc/grid_sizes.h (X(n) entries)."""
+    import re
+    with open(os.path.join(CSRC, "grid_sizes.h")) as f:
+        return [int(m) for m in re.findall(r"X\((\d+)\)", f.read())]
+
+
+def units():
+    """(source, object, extra flags) of every translation unit: inst.hip
+    once per grid size, every other .hip source once."""
+    out = []
+    for f in sorted(os.listdir(CSRC)):
+        if not f.endswith(".hip"):
+            continue
+        src = os.path.join(CSRC, f)
+        if f == "inst.hip":
+            # largest grids first: they take longest to compile
+            for n in sorted(grid_sizes(), reverse=True):
+                out.append((src, os.path.join(OBJ, f"inst_{n}.o"), [f"-DDNS_N={n}"]))
+        else:
+            out.append((src, os.path.join(OBJ, f[:-4] + ".o"), []))
+    return out
 
 
 def build_library(jobs=None, verbose=False):
     os.makedirs(OBJ, exist_ok=True)
-    srcs = sources()
     jobs = jobs or min(8, os.cpu_count() or 1)
 
-    def compile_one(src):
-        obj = os.path.join(OBJ, os.path.basename(src)[:-4] + ".o")
-        _run([HIPCC, *HIP_FLAGS, "-c", src, "-o", obj])
+    def compile_one(u):
+        src, obj, extra = u
+        _run([HIPCC, *HIP_FLAGS, *extra, "-c", src, "-o", obj])
         return obj
 
     with cf.ThreadPoolExecutor(jobs) as ex:
-        objs = list(ex.map(compile_one, srcs))
+        objs = list(ex.map(compile_one, units()))
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)
@@ -68,7 +87,7 @@ def build_tools(verbose=False):
     inc = os.path.join(ROOT, "include")
     cflags = ["-std=gnu99", "-O2", "-Wall", "-Wextra", f"-I{inc}"]
     dns = os.path.join(BIN, "dns")
-    _run([CC, *cflags, os.path.join(PKG, "cli", "dns.c"), f"-L{PKG}", "-ldns_hip",
+    _run([CC, *cflags, "-pthread", os.path.join(PKG, "cli", "dns.c"), f"-L{PKG}", "-ldns_hip",
           "-Wl,-rpath,$ORIGIN/..", "-lm", "-o", dns])
     ic = os.path.join(BIN, "dns-ic")
     _run([CC, *cflags, os.path.join(PKG, "cli", "dns_ic.c"), f"-L{PKG}", "-ldns_hip",

@@ -16,14 +16,24 @@
  * The reference dies with SIGFPE at the first overflow/invalid operation
  * (main.c:49-50); here the diagnostics are checked every 10 steps and a
  * non-finite energy or enstrophy raises SIGFPE before its line is printed.
+ *
+ * Snapshots (-d) are written while the next steps run: the transforms and
+ * the device-to-host copy are queued (dns_snapshot_begin), a writer thread
+ * waits for the copy and writes the raw file (in parallel chunks) and the
+ * XDMF file, and the main thread joins it before the next diagnostics line
+ * is printed.  Files, stdout, error messages and exit codes are those of
+ * the synchronous dump of main.c:269-330.
  */
 #define _GNU_SOURCE
+#include <fcntl.h>
 #include <math.h>
+#include <pthread.h>
 #include <signal.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
+#include <unistd.h>
 #include "dns.h"
 
 enum { nvars = 4 };
@@ -85,15 +95,12 @@ static void nonfinite(void) {
 
 /* main.c:281-328, including its quirks: the file is numbered by idump,
  * it references "<idump>.raw", and the geometry type is spelt as there. */
-static void write_xdmf(long idump, double t, long n, double dx) {
-  char path[FILENAME_MAX];
+static int write_xdmf(long idump, double t, long n, double dx, char *path) {
   static const char *names[nvars] = {"U", "V", "W", "P"};
   sprintf(path, "a.%08ld.xdmf2", idump);
   FILE *file = fopen(path, "w");
-  if (file == NULL) {
-    fprintf(stderr, "dns: error: fail to open '%s'\n", path);
-    exit(1);
-  }
+  if (file == NULL)
+    return -1;
   fprintf(file,
           "<Xdmf\n"
           "    Version=\"2\">\n"
@@ -140,6 +147,95 @@ static void write_xdmf(long idump, double t, long n, double dx) {
                 "  </Domain>\n"
                 "</Xdmf>\n");
   fclose(file);
+  return 0;
+}
+
+/* One chunk of a raw file written with pwrite (like fwrite in main.c:275,
+ * short writes are not reported). */
+struct chunk {
+  int fd;
+  const char *p;
+  size_t len;
+  off_t off;
+};
+
+static void *write_chunk(void *arg) {
+  struct chunk *c = arg;
+  while (c->len > 0) {
+    ssize_t w = pwrite(c->fd, c->p, c->len, c->off);
+    if (w <= 0)
+      break;
+    c->p += w;
+    c->len -= (size_t)w;
+    c->off += w;
+  }
+  return NULL;
+}
+
+/* A pending snapshot: written by dump_writer, checked by dump_finish. */
+struct dump_job {
+  dns_ctx *ctx;
+  long tstep, idump, n;
+  double t, dx;
+  int rc;                    /* library error of dns_snapshot_wait */
+  int open_failed;           /* path could not be opened */
+  char path[FILENAME_MAX];
+};
+
+enum { max_writers = 8 };
+
+static void *dump_writer(void *arg) {
+  struct dump_job *j = arg;
+  const double *buf;
+  if ((j->rc = dns_snapshot_wait(j->ctx, &buf)) != DNS_OK)
+    return NULL;
+  sprintf(j->path, "%08ld.raw", j->tstep);
+  int fd = open(j->path, O_WRONLY | O_CREAT | O_TRUNC, 0666);
+  if (fd < 0) {
+    j->open_failed = 1;
+    return NULL;
+  }
+  const size_t bytes = (size_t)nvars * j->n * j->n * j->n * sizeof(double);
+  int nw = (int)(bytes >> 26) + 1; /* one writer per 64 MiB, at most max_writers */
+  if (nw > max_writers)
+    nw = max_writers;
+  struct chunk c[max_writers];
+  pthread_t th[max_writers];
+  const size_t per = (bytes / nw + 4095) & ~(size_t)4095;
+  int started = 0;
+  for (int i = 0; i < nw; i++) {
+    size_t off = per * i;
+    c[i].fd = fd;
+    c[i].p = (const char *)buf + (off < bytes ? off : bytes);
+    c[i].len = off < bytes ? (bytes - off < per ? bytes - off : per) : 0;
+    c[i].off = (off_t)(off < bytes ? off : bytes);
+    if (i > 0 && pthread_create(&th[i], NULL, write_chunk, &c[i]) == 0)
+      started |= 1 << i;
+    else if (i > 0)
+      write_chunk(&c[i]);
+  }
+  write_chunk(&c[0]);
+  for (int i = 1; i < nw; i++)
+    if (started & (1 << i))
+      pthread_join(th[i], NULL);
+  close(fd);
+  if (write_xdmf(j->idump, j->t, j->n, j->dx, j->path) != 0)
+    j->open_failed = 1;
+  return NULL;
+}
+
+/* Wait for the pending snapshot, if any; report its failure as main.c does. */
+static void dump_finish(pthread_t *th, int *live, struct dump_job *j) {
+  if (!*live)
+    return;
+  pthread_join(*th, NULL);
+  *live = 0;
+  if (j->rc != DNS_OK)
+    fail(j->rc);
+  if (j->open_failed) {
+    fprintf(stderr, "dns: error: fail to open '%s'\n", j->path);
+    exit(1);
+  }
 }
 
 int main(int argc, char **argv) {
@@ -231,7 +327,7 @@ int main(int argc, char **argv) {
     fprintf(stderr, "dns: n = %ld\n", n);
   }
   long n3 = n * n * n;
-  double *buf = malloc(sizeof(double) * (Dump ? nvars : 3) * n3);
+  double *buf = malloc(sizeof(double) * 3 * n3);
   if (buf == NULL) {
     fprintf(stderr, "dns: error: out of host memory\n");
     exit(1);
@@ -244,10 +340,14 @@ int main(int argc, char **argv) {
   }
   if ((rc = dns_load_velocity(ctx, buf)) != DNS_OK)
     fail(rc);
+  free(buf);
   if (Dump && (rc = dns_set_pressure_capture(ctx, 1)) != DNS_OK)
     fail(rc);
   double dx = 2 * pi / n, t = 0.0;
   long tstep = 0, idump = 0, pending = 0;
+  pthread_t writer;
+  int writer_live = 0;
+  struct dump_job job;
   const double t_start = wall_seconds();
   for (;;) { /* main.c:256-423; steps are queued and run before each line */
     if (tstep % 10 == 0) {
@@ -256,24 +356,29 @@ int main(int argc, char **argv) {
         fail(rc);
       pending = 0;
       rc = dns_diagnostics(ctx, &energy, &Omega);
+      /* the previous snapshot is complete before this line, as in main.c */
+      dump_finish(&writer, &writer_live, &job);
       if (rc == DNS_ENONFINITE)
         nonfinite();
       if (rc != DNS_OK)
         fail(rc);
       printf("% 10ld % .16e % .16Le % .16Le\n", tstep, t, (long double)energy, (long double)Omega);
       fflush(stdout);
-      if (Dump) {
-        char path[FILENAME_MAX];
-        if ((rc = dns_snapshot(ctx, buf)) != DNS_OK)
+      if (Dump) { /* main.c:269-330, written while the next steps run */
+        if ((rc = dns_snapshot_begin(ctx)) != DNS_OK)
           fail(rc);
-        sprintf(path, "%08ld.raw", tstep);
-        if ((file = fopen(path, "w")) == NULL) {
-          fprintf(stderr, "dns: error: fail to open '%s'\n", path);
+        memset(&job, 0, sizeof job);
+        job.ctx = ctx;
+        job.tstep = tstep;
+        job.idump = idump;
+        job.n = n;
+        job.t = t;
+        job.dx = dx;
+        if (pthread_create(&writer, NULL, dump_writer, &job) != 0) {
+          fprintf(stderr, "dns: error: cannot start the snapshot writer\n");
           exit(1);
         }
-        fwrite(buf, n3 * nvars, sizeof(double), file);
-        fclose(file);
-        write_xdmf(idump, t, n, dx);
+        writer_live = 1;
         idump++;
       }
     }
@@ -283,6 +388,7 @@ int main(int argc, char **argv) {
     t += dt;
     tstep++;
   }
+  dump_finish(&writer, &writer_live, &job);
   if (pending) { /* the reference runs these steps too before exiting */
     double e, z;
     if ((rc = dns_step(ctx, pending)) != DNS_OK)
@@ -295,7 +401,6 @@ int main(int argc, char **argv) {
     const double el = wall_seconds() - t_start;
     fprintf(stderr, "dns: %ld steps in %.3f s (%.2f steps/s)\n", tstep, el, el > 0 ? tstep / el : 0.0);
   }
-  free(buf);
   dns_destroy(ctx);
   return 0;
 }

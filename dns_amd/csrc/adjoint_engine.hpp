@@ -34,7 +34,7 @@ struct AdjWork {
 template <int N> struct Adjoint {
   static constexpr int E = Sched<N>::E, TPP = N / E;
   static constexpr int T = 8 * TPP > 512 ? 512 / TPP : 8;
-  static constexpr int PPW = TPP >= 64 ? 1 : 64 / TPP;
+  static constexpr int PPW = z_pencils_per_block<N>();
   static constexpr int NF = N / 2 + 1;
   static dim3 gx() { return dim3((NF + T - 1) / T, N); }
 

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <new>
 #include "engine.hpp"
+#include "grid_sizes.h"
 
 namespace dnsk {
 
@@ -15,34 +16,20 @@ void set_error(const char *what, hipError_t e) {
   snprintf(g_err, sizeof g_err, "%s: %s", what, hipGetErrorString(e));
 }
 
-// One translation unit per grid size (inst_<n>.hip) instantiates the engine.
-Impl *make_engine_16(int);
-Impl *make_engine_32(int);
-Impl *make_engine_48(int);
-Impl *make_engine_64(int);
-Impl *make_engine_96(int);
-Impl *make_engine_128(int);
-Impl *make_engine_192(int);
-Impl *make_engine_256(int);
-Impl *make_engine_384(int);
-Impl *make_engine_512(int);
-Impl *make_engine_768(int);
-Impl *make_engine_1024(int);
+void set_error_text(const char *what) { snprintf(g_err, sizeof g_err, "%s", what); }
+
+// One translation unit per grid size (inst.hip compiled with -DDNS_N=<n>)
+// instantiates the engine.
+#define DNS_DECLARE(n) Impl *make_engine_##n(int);
+DNS_GRID_SIZES(DNS_DECLARE)
+#undef DNS_DECLARE
 
 static Impl *make_impl(int n, int prec) {
   switch (n) {
-    case 16: return make_engine_16(prec);
-    case 32: return make_engine_32(prec);
-    case 48: return make_engine_48(prec);
-    case 64: return make_engine_64(prec);
-    case 96: return make_engine_96(prec);
-    case 128: return make_engine_128(prec);
-    case 192: return make_engine_192(prec);
-    case 256: return make_engine_256(prec);
-    case 384: return make_engine_384(prec);
-    case 512: return make_engine_512(prec);
-    case 768: return make_engine_768(prec);
-    case 1024: return make_engine_1024(prec);
+#define DNS_CASE(n) \
+  case n: return make_engine_##n(prec);
+    DNS_GRID_SIZES(DNS_CASE)
+#undef DNS_CASE
     default: return nullptr;
   }
 }
@@ -241,9 +228,12 @@ struct dns_batch {
   hipStream_t main;
   hipEvent_t fork;
   std::vector<hipEvent_t> join;
-  hipGraph_t graph;
-  hipGraphExec_t exec;
+  // graphs of BATCH_GRAPH_STEPS steps and of one step of all groups
+  hipGraphExec_t exec, exec1;
 };
+
+// Steps per replayed batch graph (one graph launch per this many steps).
+static const long BATCH_GRAPH_STEPS = 10;
 
 extern "C" {
 
@@ -282,7 +272,14 @@ void dns_destroy(dns_ctx *c) {
   for (hipEvent_t e : c->ev_pool)
     if (e) hipEventDestroy(e);
   delete (AdjWork *)c->adj;
+  if (c->snap_stream) hipStreamSynchronize(c->snap_stream);
+  if (c->snap_dev) hipFree(c->snap_dev);
+  if (c->snap_host) hipHostFree(c->snap_host);
+  if (c->snap_ready) hipEventDestroy(c->snap_ready);
+  if (c->snap_done) hipEventDestroy(c->snap_done);
+  if (c->snap_stream) hipStreamDestroy(c->snap_stream);
   if (c->gexec) hipGraphExecDestroy(c->gexec);
+  if (c->gexec1) hipGraphExecDestroy(c->gexec1);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c->impl;
   delete c;
@@ -326,6 +323,40 @@ int dns_snapshot(dns_ctx *c, double *uvwp) {
   if (!c || !uvwp) return DNS_EINVAL;
   DNS_HIP(hipSetDevice(c->device));
   return c->impl->snapshot(c, 0, uvwp);
+}
+
+int dns_snapshot_begin(dns_ctx *c) {
+  if (!c) return DNS_EINVAL;
+  DNS_HIP(hipSetDevice(c->device));
+  const size_t bytes = 4 * sizeof(double) * (size_t)c->g.rs;
+  if (!c->snap_dev) {
+    DNS_HIP(hipStreamCreateWithFlags(&c->snap_stream, hipStreamNonBlocking));
+    DNS_HIP(hipEventCreateWithFlags(&c->snap_ready, hipEventDisableTiming));
+    DNS_HIP(hipEventCreateWithFlags(&c->snap_done, hipEventDisableTiming));
+    DNS_HIP(hipHostMalloc((void **)&c->snap_host, bytes, hipHostMallocDefault));
+    void *d = nullptr;
+    DNS_HIP(hipMalloc(&d, bytes));
+    c->snap_dev = (double *)d;
+    c->bytes += (long long)bytes;
+  }
+  // the staging buffer is reused: the previous copy out of it must be done
+  if (c->snap_pending) DNS_HIP(hipStreamWaitEvent(c->stream, c->snap_done, 0));
+  int rc = c->impl->snapshot_device(c, c->snap_dev);
+  if (rc) return rc;
+  DNS_HIP(hipEventRecord(c->snap_ready, c->stream));
+  DNS_HIP(hipStreamWaitEvent(c->snap_stream, c->snap_ready, 0));
+  DNS_HIP(hipMemcpyAsync(c->snap_host, c->snap_dev, bytes, hipMemcpyDeviceToHost, c->snap_stream));
+  DNS_HIP(hipEventRecord(c->snap_done, c->snap_stream));
+  c->snap_pending = 1;
+  return DNS_OK;
+}
+
+int dns_snapshot_wait(dns_ctx *c, const double **uvwp) {
+  if (!c || !uvwp || !c->snap_pending) return DNS_EINVAL;
+  DNS_HIP(hipSetDevice(c->device));
+  DNS_HIP(hipEventSynchronize(c->snap_done));
+  *uvwp = c->snap_host;
+  return DNS_OK;
 }
 
 int dns_get_spectral(dns_ctx *c, double *uvw_hat) {
@@ -458,7 +489,7 @@ void dns_batch_destroy(dns_batch *b) {
   if (!b) return;
   hipSetDevice(b->device);
   if (b->exec) hipGraphExecDestroy(b->exec);
-  if (b->graph) hipGraphDestroy(b->graph);
+  if (b->exec1) hipGraphExecDestroy(b->exec1);
   if (b->main) hipStreamSynchronize(b->main);
   for (dns_ctx *c : b->groups) dns_destroy(c);
   for (hipEvent_t e : b->join) hipEventDestroy(e);
@@ -527,6 +558,36 @@ int dns_batch_load_velocity(dns_batch *b, int member, const double *uvw) {
   return c->impl->load(c, i, uvw);
 }
 
+}  // extern "C"
+
+// Capture k steps of every group into one graph, one concurrent branch per
+// group (fork/join events on the batch's own stream).
+static int batch_capture(dns_batch *b, long k, hipGraphExec_t *out) {
+  hipGraph_t graph = nullptr;
+  DNS_HIP(hipStreamBeginCapture(b->main, hipStreamCaptureModeGlobal));
+  int rc = DNS_OK;
+  hipError_t e = hipEventRecord(b->fork, b->main);
+  for (size_t gi = 0; gi < b->groups.size() && rc == DNS_OK && e == hipSuccess; gi++) {
+    dns_ctx *c = b->groups[gi];
+    e = hipStreamWaitEvent(c->stream, b->fork, 0);
+    if (e == hipSuccess) rc = c->impl->step(c, k);
+    if (rc == DNS_OK && e == hipSuccess) e = hipEventRecord(b->join[gi], c->stream);
+    if (rc == DNS_OK && e == hipSuccess) e = hipStreamWaitEvent(b->main, b->join[gi], 0);
+  }
+  hipError_t e2 = hipStreamEndCapture(b->main, &graph);
+  if (rc || e != hipSuccess || e2 != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    if (rc) return rc;
+    DNS_HIP(e != hipSuccess ? e : e2);
+  }
+  e = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  DNS_HIP(e);
+  return DNS_OK;
+}
+
+extern "C" {
+
 int dns_batch_step(dns_batch *b, long nsteps) {
   if (!b || nsteps < 0) return DNS_EINVAL;
   if (nsteps == 0) return DNS_OK;
@@ -538,26 +599,12 @@ int dns_batch_step(dns_batch *b, long nsteps) {
     if (rc) return rc;
     DNS_HIP(hipStreamSynchronize(c->stream));
   }
-  if (!b->exec) {  // capture one step of all groups, one branch per group
-    DNS_HIP(hipStreamBeginCapture(b->main, hipStreamCaptureModeGlobal));
-    DNS_HIP(hipEventRecord(b->fork, b->main));
-    for (size_t gi = 0; gi < b->groups.size(); gi++) {
-      dns_ctx *c = b->groups[gi];
-      DNS_HIP(hipStreamWaitEvent(c->stream, b->fork, 0));
-      int rc = c->impl->step(c, 1);
-      if (rc) {
-        hipGraph_t g;
-        hipStreamEndCapture(b->main, &g);
-        if (g) hipGraphDestroy(g);
-        return rc;
-      }
-      DNS_HIP(hipEventRecord(b->join[gi], c->stream));
-      DNS_HIP(hipStreamWaitEvent(b->main, b->join[gi], 0));
-    }
-    DNS_HIP(hipStreamEndCapture(b->main, &b->graph));
-    DNS_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
-  }
-  for (long s = 0; s < nsteps; s++) DNS_HIP(hipGraphLaunch(b->exec, b->main));
+  const long bulk = nsteps / BATCH_GRAPH_STEPS, rest = nsteps % BATCH_GRAPH_STEPS;
+  int rc;
+  if (bulk && !b->exec && (rc = batch_capture(b, BATCH_GRAPH_STEPS, &b->exec))) return rc;
+  if (rest && !b->exec1 && (rc = batch_capture(b, 1, &b->exec1))) return rc;
+  for (long s = 0; s < bulk; s++) DNS_HIP(hipGraphLaunch(b->exec, b->main));
+  for (long s = 0; s < rest; s++) DNS_HIP(hipGraphLaunch(b->exec1, b->main));
   // the groups' own streams must see the graph's work before later calls
   DNS_HIP(hipEventRecord(b->fork, b->main));
   for (dns_ctx *c : b->groups) DNS_HIP(hipStreamWaitEvent(c->stream, b->fork, 0));
@@ -570,14 +617,10 @@ int dns_batch_autotune(dns_batch *b, int steps) {
     int rc = ctx_autotune(c, steps, nullptr, nullptr);
     if (rc) return rc;
   }
-  if (b->exec) {  // recapture with the chosen shapes
-    hipGraphExecDestroy(b->exec);
-    b->exec = nullptr;
-  }
-  if (b->graph) {
-    hipGraphDestroy(b->graph);
-    b->graph = nullptr;
-  }
+  // recapture with the chosen shapes
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->exec1) (void)hipGraphExecDestroy(b->exec1);
+  b->exec = b->exec1 = nullptr;
   return DNS_OK;
 }
 

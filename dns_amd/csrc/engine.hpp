@@ -33,6 +33,7 @@
 namespace dnsk {
 
 void set_error(const char *what, hipError_t e);
+void set_error_text(const char *what);
 
 #define DNS_HIP(x)                                                   \
   do {                                                               \
@@ -75,7 +76,8 @@ struct dns_ctx {
   void *adj;  // adjoint workspace (dnsk::AdjWork), allocated on first use
   int variant;            // launch shape (see Engine::step_launch)
   int use_graph;          // replay steps from a captured graph
-  hipGraphExec_t gexec;   // graph of one step for `gvariant`
+  // graphs of GRAPH_STEPS steps and of one step for launch shape `gvariant`
+  hipGraphExec_t gexec, gexec1;
   int gvariant;
   // HIP-event timing of the step's kernels on `stream` (off by default)
   int timing;
@@ -87,6 +89,13 @@ struct dns_ctx {
   std::vector<TimedLaunch> ev_pending;
   double slot_ms[DNS_NSLOT];
   long slot_n[DNS_NSLOT];
+  // asynchronous snapshots: device staging [4][n][n][n] doubles, its pinned
+  // host copy, the copy stream and the events that order them (allocated by
+  // the first dns_snapshot_begin)
+  double *snap_dev, *snap_host;
+  hipStream_t snap_stream;
+  hipEvent_t snap_ready, snap_done;
+  int snap_pending;
 };
 
 namespace dnsk {
@@ -106,6 +115,9 @@ struct Impl {
   virtual int variants() const = 0;
   virtual int set_variant(dns_ctx *c, int v) = 0;
   virtual int snapshot(dns_ctx *c, int m, double *uvwp) = 0;
+  // c2r(U,V,W,P)/n^3 of member 0 into `dst` (device [4][n][n][n] doubles)
+  // on the context's stream, asynchronously
+  virtual int snapshot_device(dns_ctx *c, double *dst) = 0;
   virtual int rfft3(dns_ctx *c, const double *in, double *out) = 0;
   virtual int irfft3(dns_ctx *c, const double *in, double *out) = 0;
   virtual int get_us(dns_ctx *c, int m, double *out) = 0;
@@ -195,7 +207,7 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
   // y passes: double grids whose pencils need more than 64 threads (n=1024)
   // take 16 elements per thread, keeping 8-column tiles in 512 threads
   // (n=1024: inverse 25.0 -> 22.5 ms, forward 7.7 -> 5.7 ms).
-  static constexpr int E_Y = (sizeof(R) == 8 && TPP > 64) ? 2 * E : E;
+  static constexpr int E_Y = (sizeof(R) == 8 && TPP > 64 && N % (2 * E) == 0) ? 2 * E : E;
   static constexpr int TPP_Y = N / E_Y;
   // 8 double / 16 float columns (float y passes use up to 1024 threads;
   // narrower float tiles over-fetch: 64-byte segments read 1.8x their bytes)
@@ -228,9 +240,10 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
   static constexpr int EPI_X = (sizeof(SR) == 4 && E_X == 16) ? 4 : 0;
   static constexpr int EPI_X3 = EPI_X;
   static constexpr int T_X = sizeof(R) == 8 ? T_D : (16 * TPP_X > 512 ? 512 / TPP_X : 16);
-  // z pencils per block: one wave (or one pencil when a pencil needs more
-  // than a wave), so the pass's barriers never couple independent pencils.
-  static constexpr int PPW = TPP >= 64 ? 1 : 64 / TPP;
+  // z pencils per block: at most one wave (or one pencil when a pencil needs
+  // more than a wave), so the pass's barriers never couple independent
+  // pencils across waves.
+  static constexpr int PPW = z_pencils_per_block<N>();
   static constexpr int NF = N / 2 + 1;
 
   size_t sr_size() const override { return sizeof(cx<SR>); }
@@ -357,7 +370,10 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
   static constexpr int T_YH = T_Y >= 4 ? T_Y / 2 : T_Y;
   static constexpr int T_YFH = T_YF >= 4 ? T_YF / 2 : T_YF;
   static constexpr int T_XH = T_X >= 4 ? T_X / 2 : T_X;
-  static constexpr int PPW2 = 2 * PPW;
+  // two waves of pencils; where a wave's pencils do not fill it exactly
+  // (64 % TPP != 0) a second set would straddle the wave boundary, which the
+  // lane exchanges of the Hermitian loads do not allow: one set only
+  static constexpr int PPW2 = (TPP > 64 || 64 % TPP == 0) && (N * N) % (2 * PPW) == 0 ? 2 * PPW : PPW;
   int variants() const override { return TUNABLE ? 4 : 1; }
 
   int step(dns_ctx *c, long nsteps) override {
@@ -379,28 +395,41 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
     return step_body<T_Y, T_YF, T_X, PPW>(c, nsteps);
   }
 
-  // One step captured once into a graph (per launch shape) and replayed.
-  int step_graph(dns_ctx *c, long nsteps) {
-    if (c->gexec && c->gvariant != c->variant) {
-      hipGraphExecDestroy(c->gexec);
-      c->gexec = nullptr;
-    }
-    if (!c->gexec) {
-      hipGraph_t graph = nullptr;
-      DNS_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      int rc = step_launch(c, 1);
-      hipError_t e = hipStreamEndCapture(c->stream, &graph);
-      if (rc || e != hipSuccess) {
-        if (graph) hipGraphDestroy(graph);
-        if (rc) return rc;
-        DNS_HIP(e);
-      }
-      e = hipGraphInstantiate(&c->gexec, graph, nullptr, nullptr, 0);
-      hipGraphDestroy(graph);
+  // Steps replayed from captured graphs (per launch shape): one of
+  // GRAPH_STEPS steps for the bulk of a call, one of a single step for the
+  // remainder, so a call of many steps costs one graph launch per
+  // GRAPH_STEPS steps.
+  static constexpr long GRAPH_STEPS = 10;
+
+  int capture(dns_ctx *c, long k, hipGraphExec_t *out) {
+    hipGraph_t graph = nullptr;
+    DNS_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = step_launch(c, k);
+    hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    if (rc || e != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      if (rc) return rc;
       DNS_HIP(e);
+    }
+    e = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    DNS_HIP(e);
+    return DNS_OK;
+  }
+
+  int step_graph(dns_ctx *c, long nsteps) {
+    if (c->gvariant != c->variant) {
+      if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+      if (c->gexec1) (void)hipGraphExecDestroy(c->gexec1);
+      c->gexec = c->gexec1 = nullptr;
       c->gvariant = c->variant;
     }
-    for (long i = 0; i < nsteps; i++) DNS_HIP(hipGraphLaunch(c->gexec, c->stream));
+    const long bulk = nsteps / GRAPH_STEPS, rest = nsteps % GRAPH_STEPS;
+    int rc;
+    if (bulk && !c->gexec && (rc = capture(c, GRAPH_STEPS, &c->gexec))) return rc;
+    if (rest && !c->gexec1 && (rc = capture(c, 1, &c->gexec1))) return rc;
+    for (long i = 0; i < bulk; i++) DNS_HIP(hipGraphLaunch(c->gexec, c->stream));
+    for (long i = 0; i < rest; i++) DNS_HIP(hipGraphLaunch(c->gexec1, c->stream));
     c->w_ready = 1;
     return DNS_OK;
   }
@@ -498,6 +527,24 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
       if (rc) return rc;
       DNS_HIP(hipMemcpyAsync(uvwp + f * c->g.rs, tmpr, rb, hipMemcpyDeviceToHost, s));
       DNS_HIP(hipStreamSynchronize(s));
+    }
+    return DNS_OK;
+  }
+
+  int snapshot_device(dns_ctx *c, double *dst) override {
+    c->w_ready = 0;
+    hipStream_t s = c->stream;
+    double2 *tmpc = (double2 *)wslice(c, 0);
+    const size_t rb = sizeof(double) * c->g.rs;
+    for (int f = 0; f < 4; f++) {
+      double *out = dst + f * c->g.rs;
+      if (f == 3 && !c->have_phat) {
+        DNS_HIP(hipMemsetAsync(out, 0, rb, s));
+        continue;
+      }
+      const double2 *src = f < 3 ? (const double2 *)c->uhn + f * c->g.fs : (const double2 *)c->phat;
+      int rc = inv3_double(c, src, tmpc, out, c->invn3);
+      if (rc) return rc;
     }
     return DNS_OK;
   }

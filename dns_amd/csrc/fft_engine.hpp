@@ -88,20 +88,47 @@ template <typename R> __device__ __forceinline__ void st_real(R *p, R v) {
 }
 
 // ---------------------------------------------------------------------------
-// Radix schedules.  E = elements per thread (divisible by every radix).
+// Radix schedules.  E = elements per thread (divisible by every radix), so
+// a pencil takes N/E threads; radix-3 grids keep E at 6 or 12 (registers).
 template <int N> struct Sched;
-template <> struct Sched<16> { static constexpr int E = 4, NS = 2; static constexpr int r[NS] = {4, 4}; };
-template <> struct Sched<32> { static constexpr int E = 8, NS = 2; static constexpr int r[NS] = {8, 4}; };
-template <> struct Sched<48> { static constexpr int E = 12, NS = 3; static constexpr int r[NS] = {4, 4, 3}; };
-template <> struct Sched<64> { static constexpr int E = 8, NS = 2; static constexpr int r[NS] = {8, 8}; };
-template <> struct Sched<96> { static constexpr int E = 12, NS = 4; static constexpr int r[NS] = {4, 4, 2, 3}; };
-template <> struct Sched<128> { static constexpr int E = 8, NS = 3; static constexpr int r[NS] = {8, 4, 4}; };
-template <> struct Sched<192> { static constexpr int E = 12, NS = 4; static constexpr int r[NS] = {4, 4, 4, 3}; };
-template <> struct Sched<256> { static constexpr int E = 8, NS = 3; static constexpr int r[NS] = {8, 8, 4}; };
-template <> struct Sched<384> { static constexpr int E = 12, NS = 5; static constexpr int r[NS] = {4, 4, 4, 2, 3}; };
-template <> struct Sched<512> { static constexpr int E = 8, NS = 3; static constexpr int r[NS] = {8, 8, 8}; };
-template <> struct Sched<768> { static constexpr int E = 12, NS = 5; static constexpr int r[NS] = {4, 4, 4, 4, 3}; };
-template <> struct Sched<1024> { static constexpr int E = 8, NS = 4; static constexpr int r[NS] = {8, 8, 4, 4}; };
+#define DNS_SCHED(N_, E_, ...)                                \
+  template <> struct Sched<N_> {                              \
+    static constexpr int E = E_;                              \
+    static constexpr int r[] = {__VA_ARGS__};                 \
+    static constexpr int NS = sizeof(r) / sizeof(r[0]);       \
+  };
+DNS_SCHED(8, 4, 4, 2)
+DNS_SCHED(12, 12, 4, 3)
+DNS_SCHED(16, 4, 4, 4)
+DNS_SCHED(18, 6, 2, 3, 3)
+DNS_SCHED(24, 12, 4, 2, 3)
+DNS_SCHED(32, 8, 8, 4)
+DNS_SCHED(36, 12, 4, 3, 3)
+DNS_SCHED(48, 12, 4, 4, 3)
+DNS_SCHED(54, 6, 2, 3, 3, 3)
+DNS_SCHED(64, 8, 8, 8)
+DNS_SCHED(72, 12, 4, 2, 3, 3)
+DNS_SCHED(96, 12, 4, 4, 2, 3)
+DNS_SCHED(108, 12, 4, 3, 3, 3)
+DNS_SCHED(128, 8, 8, 4, 4)
+DNS_SCHED(144, 12, 4, 4, 3, 3)
+DNS_SCHED(162, 6, 2, 3, 3, 3, 3)
+DNS_SCHED(192, 12, 4, 4, 4, 3)
+DNS_SCHED(216, 12, 4, 2, 3, 3, 3)
+DNS_SCHED(256, 8, 8, 8, 4)
+DNS_SCHED(288, 12, 4, 4, 2, 3, 3)
+DNS_SCHED(324, 12, 4, 3, 3, 3, 3)
+DNS_SCHED(384, 12, 4, 4, 4, 2, 3)
+DNS_SCHED(432, 12, 4, 4, 3, 3, 3)
+DNS_SCHED(486, 6, 2, 3, 3, 3, 3, 3)
+DNS_SCHED(512, 8, 8, 8, 8)
+DNS_SCHED(576, 12, 4, 4, 4, 3, 3)
+DNS_SCHED(648, 12, 4, 2, 3, 3, 3, 3)
+DNS_SCHED(768, 12, 4, 4, 4, 4, 3)
+DNS_SCHED(864, 12, 4, 4, 2, 3, 3, 3)
+DNS_SCHED(972, 12, 4, 3, 3, 3, 3, 3)
+DNS_SCHED(1024, 8, 8, 8, 4, 4)
+#undef DNS_SCHED
 
 template <int N> constexpr int stage_Ns(int s) {
   int ns = 1;

@@ -51,6 +51,16 @@ struct Geo {
   long wfs, wps;
 };
 
+// z pencils per block of the contiguous-axis passes: as many as fill one
+// wave (one pencil when a pencil needs a wave or more), reduced to a divisor
+// of the n*n rows so that every block is full.
+template <int N> constexpr int z_pencils_per_block() {
+  constexpr int TPP = N / Sched<N>::E;
+  int p = TPP >= 64 ? 1 : 64 / TPP;
+  while ((N * N) % p) p--;
+  return p;
+}
+
 // Index i of an x or y axis lies inside the dealiased box.
 __device__ __forceinline__ bool in_box(int i, int n, int nk) { return i < nk || i > n - nk; }
 
@@ -857,6 +867,9 @@ __global__ __launch_bounds__(T *(N / E), MINW) void k_xfwd_rk_xinv(cx<R> *__rest
 //   e = |U|^2 + |V|^2 + |W|^2 (double, left to right), z = kk * e,
 // summed with compensated (two-sum) accumulation per thread, then a
 // compensated tree per block; partial sums go to part[4*block].
+// XMAJOR: the three fields are in the x-major layout of the host API
+// ((i*n + j)*nfp + k) instead of the state layout; the elements are visited
+// in the same order either way, so both give bitwise the same sums.
 __device__ __forceinline__ void two_sum_acc(double &s, double &c, double x) {
   const double t = s + x;
   const double bp = t - s;
@@ -864,7 +877,7 @@ __device__ __forceinline__ void two_sum_acc(double &s, double &c, double x) {
   s = t;
 }
 
-template <int BS>
+template <int BS, bool XMAJOR = false>
 __global__ __launch_bounds__(BS) void k_diag(const double2 *__restrict__ uh, int n, Geo g,
                                              double *__restrict__ part) {
 #pragma clang fp contract(off)
@@ -876,7 +889,7 @@ __global__ __launch_bounds__(BS) void k_diag(const double2 *__restrict__ uh, int
     int i, j;
     state_row(row, n, i, j);
     const double kx = i < n / 2 ? i : i - n, ky = j < n / 2 ? j : j - n;
-    const double2 *r = uh + row * g.nfp;
+    const double2 *r = uh + (XMAJOR ? (long)i * n + j : row) * g.nfp;
     for (int k = threadIdx.x; k < nf; k += BS) {
       const double2 a = r[k], b = r[g.fs + k], c = r[2 * g.fs + k];
       const double e = (a.x * a.x + a.y * a.y) + (b.x * b.x + b.y * b.y) + (c.x * c.x + c.y * c.y);
@@ -910,10 +923,13 @@ __global__ __launch_bounds__(BS) void k_diag(const double2 *__restrict__ uh, int
   }
 }
 
-// Final compensated reduction of `nb` block partials by one block.
-template <int BS>
+// Final compensated reduction of `nb` block partials by one block: out[0..3]
+// = (energy sum, its correction, enstrophy sum, its correction), or with
+// SCALED out[0..1] = ((sum + correction) * s2) of each, s2 = (1/n^3)^2, the
+// same operations the host applies to the four values (main.c:265-266).
+template <int BS, bool SCALED = false>
 __global__ __launch_bounds__(BS) void k_diag_final(const double *__restrict__ part, int nb,
-                                                   double *__restrict__ out) {
+                                                   double *__restrict__ out, double s2 = 0.0) {
 #pragma clang fp contract(off)
   part += 4L * blockIdx.z * nb;
   out += 4 * blockIdx.z;
@@ -938,10 +954,15 @@ __global__ __launch_bounds__(BS) void k_diag_final(const double *__restrict__ pa
       two_sum_acc(z, zcc, sh[2][q]);
       zcc += sh[3][q];
     }
-    out[0] = a;
-    out[1] = ac;
-    out[2] = z;
-    out[3] = zcc;
+    if (SCALED) {
+      out[0] = (a + ac) * s2;
+      out[1] = (z + zcc) * s2;
+    } else {
+      out[0] = a;
+      out[1] = ac;
+      out[2] = z;
+      out[3] = zcc;
+    }
   }
 }
 

@@ -40,10 +40,24 @@ def n_box(n):
     return m
 
 
+def elements_per_thread(n):
+    """Elements per thread E of n's radix schedule (DNS_SCHED entries of
+    csrc/fft_engine.hpp: a pencil takes n / E threads)."""
+    import os
+    import re
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "csrc", "fft_engine.hpp")
+    with open(path) as f:
+        table = {int(a): int(b) for a, b in re.findall(r"DNS_SCHED\((\d+),\s*(\d+)", f.read())}
+    return table[n]
+
+
 def curlx_in_y(n, precision="fp64"):
-    """Whether the inverse y pass forms omega_x (see the table above)."""
-    e = 12 if n % 3 == 0 else (4 if n == 16 else 8)   # elements per thread of the schedule
-    return not (precision == "fp64" and n // e > 64)
+    """Whether the inverse y pass forms omega_x (see the table above): not
+    where FP64 y pencils take twice the schedule's elements per thread (a
+    pencil over more than 64 threads, when n divides by 2E)."""
+    e = elements_per_thread(n)
+    wide = precision == "fp64" and n // e > 64 and n % (2 * e) == 0
+    return not wide
 
 
 def slot_bytes(n, precision="fp64"):

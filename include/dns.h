@@ -53,8 +53,10 @@ enum dns_precision { DNS_FP64 = 0, DNS_MIXED = 1 };
 
 typedef struct dns_ctx dns_ctx;
 
-/* Grid n x n x n with n = 2^a 3^b, 16 <= n <= 1024 (n=16,32,48,64,96,128,
- * 192,256,384,512,768,1024).  nu = viscosity (-n), dt = time step (-s).
+/* Grid n x n x n with n even, n = 2^a 3^b and 8 <= n <= 1024 (31 sizes:
+ * 8, 12, 16, 18, 24, 32, 36, 48, 54, 64, 72, 96, 108, 128, 144, 162, 192,
+ * 216, 256, 288, 324, 384, 432, 486, 512, 576, 648, 768, 864, 972, 1024);
+ * other n give DNS_EINVAL.  nu = viscosity (-n), dt = time step (-s).
  * device = HIP device ordinal.  On success *out holds a new context. */
 int dns_create(dns_ctx **out, int n, double nu, double dt, int precision, int device);
 void dns_destroy(dns_ctx *ctx);
@@ -85,6 +87,21 @@ int dns_diagnostics(dns_ctx *ctx, double *energy, double *enstrophy);
 /* Host [4][n][n][n] doubles: c2r(U,V,W,P)/n^3 (main.c:269-280).  P is the
  * captured stage-4 pressure, or zeros if none was captured. */
 int dns_snapshot(dns_ctx *ctx, double *uvwp);
+
+/* Asynchronous snapshot, the same values as dns_snapshot (main.c:269-280).
+ * dns_snapshot_begin queues the four c2r transforms into a device staging
+ * buffer on the context's stream and their copy into a pinned host buffer
+ * owned by the context on a second stream, and returns at once: dns_step
+ * calls made afterwards run while the copy proceeds.  dns_snapshot_wait
+ * blocks until the copy is complete and sets *uvwp to the pinned buffer
+ * ([4][n][n][n] doubles), which stays valid until the next
+ * dns_snapshot_begin or dns_destroy; it only waits on an event, so another
+ * host thread may call it while the owning thread issues dns_step.  The
+ * first call allocates the staging and pinned buffers (4 n^3 doubles each).
+ * main.c dumps synchronously; these let a caller write the files while the
+ * next steps run. */
+int dns_snapshot_begin(dns_ctx *ctx);
+int dns_snapshot_wait(dns_ctx *ctx, const double **uvwp);
 
 /* State access in the reference layouts, for checkpointing and tests.
  * Spectral: host [3][n][n][n/2+1] complex (interleaved re,im doubles).

@@ -3,9 +3,10 @@
 The reference ships the stdout of its solver for the Taylor-Green vortex
 (n=128 in directory 0128, n=256 in 0256; Re in {100,...,3000}, nu=1/Re,
 dt=0.0025, T=10, i.e. 4000 RK4 steps, one line every 10 steps; recipe in
-run.sh:1-10 and hal.sh:1-16).  This script keeps, per file, the lines of
-steps 0..400, every 100th step after that, and the final line, as data
-(step, t, energy, enstrophy) -- no reference source is copied.
+run.sh:1-10 and hal.sh:1-16).  This script keeps, as data (step, t, energy,
+enstrophy), every line of the n=128 files and, of the n=256 files, the
+lines of steps 0..400, every 100th step after that and the final line --
+no reference source is copied.
 
 It also copies the reference's digitised dissipation-rate points (img/ref.txt:
 (t, eps) pairs read off the Taylor-Green curves of Brachet et al. 1983 for
@@ -29,7 +30,7 @@ def main(root):
                 for line in f:
                     step, t, e, z = line.split()
                     step = int(step)
-                    if step <= 400 or step % 100 == 0:
+                    if n == "0128" or step <= 400 or step % 100 == 0:
                         rows.append([step, t, e, z])
             out[f"{int(n)}/{int(re_)}"] = {
                 "n": int(n), "Re": int(re_), "nu": "%.16e" % (1.0 / int(re_)),

@@ -216,7 +216,7 @@ def test_verbose_lines(tmp_path):
 
 
 def test_unsupported_grid_size(tmp_path):
-    """Grids are n = 2^a 3^b, 16..1024: a 100^3 file is read (its size
+    """Grids are even n = 2^a 3^b, 8..1024: a 100^3 file is read (its size
     passes main.c:168's check) and then refused by the library."""
     p = tmp_path / "n100.raw"
     np.zeros(4 * 100 ** 3).tofile(p)

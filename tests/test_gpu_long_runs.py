@@ -1,0 +1,71 @@
+"""Mixed precision against the FP64 GPU run over whole workloads.
+
+* Taylor-Green and the isotropic field of `dns-ic` (seed 20261202, k0=4,
+  kinetic energy 0.5) at n=256, nu=6.25e-4, dt=1e-3, 2000 RK4 steps: on
+  every diagnostics line (every 10 steps, as `dns` prints them) the mixed
+  energy is within 1e-5 relative of the FP64 energy (the bar of the mixed
+  form); the enstrophy is within 1e-5 as well, and the final physical
+  velocity within 1e-3 of max |u| in max norm (measured on these runs: 7e-8,
+  2e-7 and 6.5e-5 at most).
+* The ensemble's radix-3 grids, n=96 and n=192 (Taylor-Green and isotropic,
+  Re 1600, dt=1e-3, 1000 steps), to the same bounds.
+* Taylor-Green at n=512, nu=1/3000, dt=5e-4, 500 steps: the same bounds on
+  every tenth line."""
+import subprocess
+
+import numpy as np
+import pytest
+
+import dns_amd
+from conftest import DNS_IC, tgv_field
+
+pytestmark = pytest.mark.gpu
+
+
+def _iso(tmp_path, n, seed=20261202):
+    raw = tmp_path / f"iso{n}.raw"
+    subprocess.run([DNS_IC, "-t", "iso", "-N", str(n), "-S", str(seed), "-o", str(raw)], check=True,
+                   capture_output=True, timeout=120)
+    return np.fromfile(raw, dtype="<f8").reshape(4, n, n, n)[:3].copy()
+
+
+def _run(n, nu, dt, steps, u, precision, every=10):
+    with dns_amd.Solver(n, nu, dt, precision) as s:
+        s.load_velocity(u)
+        lines = [s.diagnostics()]
+        for _ in range(steps // every):
+            s.step(every)
+            lines.append(s.diagnostics())
+        return np.array(lines), s.get_stale_velocity()
+
+
+def _compare(n, nu, dt, steps, u, every=10):
+    (a, ua), (b, ub) = (_run(n, nu, dt, steps, u, p, every) for p in ("fp64", "mixed"))
+    de = np.max(np.abs(b[:, 0] - a[:, 0]) / a[:, 0])
+    dz = np.max(np.abs(b[:, 1] - a[:, 1]) / a[:, 1])
+    du = float(np.max(np.abs(ub - ua)) / np.max(np.abs(ua)))
+    print(f"n={n} steps={steps}: max rel dev E {de:.2e} Z {dz:.2e}, max-norm dev u {du:.2e}", flush=True)
+    assert len(a) == steps // every + 1
+    assert de <= 1e-5 and dz <= 1e-5 and du <= 1e-3, (de, dz, du)
+
+
+@pytest.mark.parametrize("ic", ["tgv", "iso"])
+def test_n256_2000_steps(tmp_path, ic):
+    n = 256
+    u = tgv_field(n)[:3] if ic == "tgv" else _iso(tmp_path, n)
+    print(f"long run {ic} n=256", flush=True)
+    _compare(n, 6.25e-4, 1e-3, 2000, u)
+
+
+@pytest.mark.parametrize("n", [96, 192])
+@pytest.mark.parametrize("ic", ["tgv", "iso"])
+def test_radix3_grids_1000_steps(tmp_path, n, ic):
+    u = tgv_field(n)[:3] if ic == "tgv" else _iso(tmp_path, n)
+    print(f"long run {ic} n={n}", flush=True)
+    _compare(n, 1.0 / 1600, 1e-3, 1000, u)
+
+
+def test_n512_500_steps():
+    n = 512
+    print("long run tgv n=512", flush=True)
+    _compare(n, 1.0 / 3000, 5e-4, 500, tgv_field(n)[:3], every=100)

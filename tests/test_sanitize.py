@@ -35,7 +35,7 @@ def bins(tmp_path_factory):
     lib = os.path.join(PKG, "libdns_hip.so")
     if os.path.exists(lib):
         for name, src in (("dns", "dns.c"), ("dns-ic", "dns_ic.c")):
-            subprocess.run([*cc, f"-I{INC}", os.path.join(PKG, "cli", src), f"-L{PKG}", "-ldns_hip",
+            subprocess.run([*cc, "-pthread", f"-I{INC}", os.path.join(PKG, "cli", src), f"-L{PKG}", "-ldns_hip",
                             f"-Wl,-rpath,{PKG}", "-lm", "-o", str(d / name)], check=True, capture_output=True)
             out[name] = str(d / name)
     return out
