@@ -187,6 +187,20 @@ int dns_batch_autotune(dns_batch *b, int steps);
 int dns_batch_launch_shape(dns_batch *b, int member);
 long long dns_batch_device_bytes(const dns_batch *b);
 
+/* Energy and enstrophy sums of dns_diagnostics (main.c:257-266) of a
+ * spectral field in caller-owned device memory, allocating nothing.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops).
+ * uvw_hat: device [3][n][n][n/2+1] complex doubles in the layout of
+ * dns_get_spectral; out: device double[2] = (energy, enstrophy), bitwise
+ * equal to what dns_diagnostics gives for the same state.  Queued on
+ * `stream` (a hipStream_t, NULL for the null stream) of the current device;
+ * the caller synchronises and checks finiteness.  n even, 2 <= n <= 1024,
+ * else DNS_EINVAL before any device work.  The block partial sums live in
+ * one fixed 32 KiB device array per GPU, shared by every call: calls that
+ * overlap on one GPU race and must be serialised by the caller. */
+int dns_diagnostics_noalloc(const double *uvw_hat, int n, void *stream, double *out);
+
 /* Bytes of device memory held by the context. */
 long long dns_device_bytes(const dns_ctx *ctx);
 
