@@ -76,6 +76,28 @@ def gpu_run(dns_amd, n, nu, dt, precision, u, warmup, steps):
             "passes": per_pass}
 
 
+def mixed_error(dns_amd, n, nu, dt, u, steps, every=10):
+    """Largest relative deviation of the mixed run's energy and enstrophy
+    from the FP64 run over every diagnostics line (every 10 steps, as dns
+    prints them) of a `steps`-step run, and the max-norm deviation of the
+    final physical velocity relative to max |u|."""
+    hist, fin = {}, {}
+    for prec in ("fp64", "mixed"):
+        with dns_amd.Solver(n, nu, dt, prec) as s:
+            s.load_velocity(u)
+            lines = [s.diagnostics()]
+            for _ in range(steps // every):
+                s.step(every)
+                lines.append(s.diagnostics())
+            hist[prec] = np.array(lines)
+            fin[prec] = s.get_stale_velocity()
+    a, b = hist["fp64"], hist["mixed"]
+    return {"steps": steps, "lines": len(a),
+            "max_rel_energy": float(np.max(np.abs(b[:, 0] - a[:, 0]) / a[:, 0])),
+            "max_rel_enstrophy": float(np.max(np.abs(b[:, 1] - a[:, 1]) / a[:, 1])),
+            "final_velocity_max_norm": float(np.max(np.abs(fin["mixed"] - fin["fp64"])) / np.max(np.abs(fin["fp64"])))}
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -149,40 +171,130 @@ def ensemble(a):
     return 0
 
 
+def _oracle_final_enstrophy(tmp, name, nu, dt, nsteps, threads):
+    """Final enstrophy of the CPU oracle's command line (dns_cpu) run of
+    `nsteps` steps on the raw file tmp/name."""
+    import subprocess
+    exe = os.path.join(ROOT, "oracle", "_build", "dns_cpu")
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    # -t between steps nsteps-1 and nsteps: the run stops right after step nsteps
+    r = subprocess.run([exe, "-i", name, "-n", repr(nu), "-s", repr(dt), "-t", repr((nsteps - 0.5) * dt)],
+                       cwd=tmp, capture_output=True, text=True, env=env, check=True)
+    last = r.stdout.split("\n")[-2].split()
+    assert int(last[0]) == nsteps, last
+    return float(last[3])
+
+
 def adjoint(a):
     """Gradient of the final enstrophy, n=128 isotropic (k0=4, E=0.5),
     nu=1/800, dt=2e-3, 200 steps, checkpoint every 20; checked against
-    central differences (eps=1e-6) of FP64 runs along 8 seeded
-    divergence-free directions."""
+    central differences (eps=1e-6) of the GPU FP64 forward along 8 seeded
+    divergence-free directions, and along the first --adj-cpu-dirs of them
+    against central differences of the FP64 CPU oracle (dns_cpu, four
+    processes of a quarter of the host's threads, run while the GPU works)."""
+    import concurrent.futures as cf
+    import tempfile
     import dns_amd
-    n, nu, dt, nsteps, every = 128, 1.0 / 800, 2e-3, a.adj_steps, 20
+    n, nu, dt, nsteps, every, eps = 128, 1.0 / 800, 2e-3, a.adj_steps, 20, 1e-6
     u0 = iso_field(n, 20261202)
+    dirs = [iso_field(n, 777 + k) for k in range(a.adj_dirs)]
+    tmp = tempfile.TemporaryDirectory()
+    pool = cf.ThreadPoolExecutor(max(1, 2 * a.adj_cpu_dirs))
+    threads = max(1, (os.cpu_count() or 4) // max(1, 2 * a.adj_cpu_dirs))
+    jobs = {}
+    for k in range(min(a.adj_cpu_dirs, a.adj_dirs)):
+        for sgn in (1, -1):
+            name = f"d{k}{sgn:+d}.raw"
+            raw = np.zeros((4, n, n, n))
+            raw[:3] = u0 + sgn * eps * dirs[k]
+            raw.tofile(os.path.join(tmp.name, name))
+            jobs[(k, sgn)] = pool.submit(_oracle_final_enstrophy, tmp.name, name, nu, dt, nsteps, threads)
     s = dns_amd.Solver(n, nu, dt, "fp64")
+    s.adjoint_forward(u0, 2, 1)  # workspace and graphs warm
+    s.adjoint_backward()
     t0 = time.perf_counter()
     J = s.adjoint_forward(u0, nsteps, every)
     t1 = time.perf_counter()
     g = s.adjoint_backward()
     t2 = time.perf_counter()
-    errs = []
-    for k in range(a.adj_dirs):
-        d = iso_field(n, 777 + k)
-        vals = []
+    errs, cpu_errs, gpu_pm = [], [], {}
+    for k, d in enumerate(dirs):
         for sgn in (1, -1):
-            s.load_velocity(u0 + sgn * 1e-6 * d)
+            s.load_velocity(u0 + sgn * eps * d)
             s.step(nsteps)
-            vals.append(s.diagnostics()[1])
-        fd = (vals[0] - vals[1]) / 2e-6
+            gpu_pm[(k, sgn)] = s.diagnostics()[1]
+        fd = (gpu_pm[(k, 1)] - gpu_pm[(k, -1)]) / (2 * eps)
         ad = float(np.sum(g * d))
         errs.append(abs(ad - fd) / abs(fd))
-        print(f"adjoint: direction {k}: adjoint {ad:.12e} fd {fd:.12e} rel {errs[-1]:.2e}", file=sys.stderr,
+        print(f"adjoint: direction {k}: adjoint {ad:.12e} GPU fd {fd:.12e} rel {errs[-1]:.2e}", file=sys.stderr,
               flush=True)
     s.close()
-    line = {"metric": "adjoint gradient of final enstrophy vs central differences", "value": max(errs),
-            "unit": "max relative error", "higher_is_better": False, "n_gpus": 1, "dtype": "f64",
-            "data": "synthetic", "J": J, "forward_s": t1 - t0, "backward_s": t2 - t1,
-            "backward_over_forward": (t2 - t1) / (t1 - t0), "rel_errors": errs,
+    fwd_dev = []
+    for k in range(min(a.adj_cpu_dirs, a.adj_dirs)):
+        cpu = {sgn: jobs[(k, sgn)].result() for sgn in (1, -1)}
+        fd = (cpu[1] - cpu[-1]) / (2 * eps)
+        ad = float(np.sum(g * dirs[k]))
+        cpu_errs.append(abs(ad - fd) / abs(fd))
+        fwd_dev += [abs(gpu_pm[(k, sgn)] - cpu[sgn]) / abs(cpu[sgn]) for sgn in (1, -1)]
+        print(f"adjoint: direction {k}: adjoint {ad:.12e} oracle fd {fd:.12e} rel {cpu_errs[-1]:.2e}",
+              file=sys.stderr, flush=True)
+    pool.shutdown()
+    tmp.cleanup()
+    line = {"metric": "adjoint gradient of final enstrophy vs central differences",
+            "value": max(errs + cpu_errs), "unit": "max relative error", "higher_is_better": False, "n_gpus": 1,
+            "dtype": "f64", "data": "synthetic", "J": J, "forward_s": t1 - t0, "backward_s": t2 - t1,
+            "backward_over_forward": (t2 - t1) / (t1 - t0),
+            "rel_errors_gpu_fd": errs, "rel_errors_oracle_fd": cpu_errs,
+            "final_enstrophy_gpu_vs_oracle_max_rel": max(fwd_dev) if fwd_dev else None,
             "config": {"workload": "gradient of final enstrophy w.r.t. initial velocity", "n": n, "nu": nu, "dt": dt,
-                       "steps": nsteps, "checkpoint_every": every, "eps": 1e-6, "directions": a.adj_dirs}}
+                       "steps": nsteps, "checkpoint_every": every, "eps": eps, "directions": a.adj_dirs,
+                       "oracle_directions": min(a.adj_cpu_dirs, a.adj_dirs)}}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def dump_overhead(a):
+    """Wall time of the drop-in command line with and without -d (snapshot
+    and XDMF files every 10 steps, written while the next steps run) on the
+    Taylor-Green field, n = --n, --steps steps; files go to --dump-dir and
+    are deleted afterwards.  Times are the command line's own -v step-loop
+    figure (diagnostics and snapshots included, reading excluded)."""
+    import re
+    import shutil
+    import subprocess
+    import tempfile
+    n, steps = a.n, a.steps
+    nu, dt = (1.0 / 3000, 5e-4) if n == 512 else (1.0 / 1600, 1e-3)
+    exe = os.path.join(ROOT, "dns_amd", "bin", "dns")
+    d = tempfile.mkdtemp(dir=a.dump_dir)
+    res = {}
+    try:
+        raw = os.path.join(d, "in.raw")
+        f = np.zeros((4, n, n, n))
+        f[:3] = tgv(n)
+        f.tofile(raw)
+        del f
+        args = ["-v", "-i", raw, "-n", repr(nu), "-s", repr(dt), "-t", repr((steps - 0.5) * dt)]
+        for mode in ("plain", "dump", "plain2", "dump2"):
+            out = os.path.join(d, mode)
+            os.mkdir(out)
+            print(f"dump: {mode} n={n} steps={steps}", file=sys.stderr, flush=True)
+            extra = ["-d"] if mode.startswith("dump") else []
+            r = subprocess.run([exe, *extra, *args], cwd=out, capture_output=True, text=True, check=True)
+            m = re.search(r"(\d+) steps in ([0-9.]+) s", r.stderr)
+            files = sorted(os.listdir(out))
+            res[mode] = {"steps": int(m.group(1)), "wall_s": float(m.group(2)), "files": len(files),
+                         "bytes": sum(os.path.getsize(os.path.join(out, x)) for x in files)}
+            shutil.rmtree(out)
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    plain = min(res["plain"]["wall_s"], res["plain2"]["wall_s"])
+    dump = min(res["dump"]["wall_s"], res["dump2"]["wall_s"])
+    line = {"metric": "dns -d wall time over dns without snapshots", "value": dump / plain, "unit": "ratio",
+            "higher_is_better": False, "n_gpus": 1, "dtype": "f64", "data": "synthetic",
+            "plain_s": plain, "dump_s": dump, "runs": res,
+            "config": {"workload": f"Taylor-Green n={n}, {steps} steps, snapshot every 10 steps", "n": n,
+                       "dump_dir": a.dump_dir}}
     print(json.dumps(line), flush=True)
     return 0
 
@@ -326,6 +438,12 @@ def main():
     ap.add_argument("--validate", action="store_true", help="n=256 golden-trajectory validation instead")
     ap.add_argument("--configs", action="store_true", help="long FP64/mixed runs at n=256 and 512 instead")
     ap.add_argument("--val-steps", type=int, default=4000)
+    ap.add_argument("--err-steps", type=int, default=500,
+                    help="steps of the FP64-vs-mixed error run (0: skip)")
+    ap.add_argument("--adj-cpu-dirs", type=int, default=2,
+                    help="directions also differenced with the CPU oracle")
+    ap.add_argument("--dump", action="store_true", help="wall time of dns -d against dns instead")
+    ap.add_argument("--dump-dir", default="/tmp", help="directory the snapshots are written to")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: runs on several GPUs are not supported (ensemble members "
@@ -340,6 +458,8 @@ def main():
         return validate(a)
     if a.configs:
         return configs(a)
+    if a.dump:
+        return dump_overhead(a)
     import dns_amd
     from dns_amd import roofline
 
@@ -356,10 +476,11 @@ def main():
                  "launch_shape": rmx["launch_shape"],
                  "step_bytes": roofline.step_bytes_fused(n, "mixed"),
                  "speedup_vs_fp64": rmx["steps_per_s"] / r64["steps_per_s"],
-                 "energy_rel_err_vs_fp64": abs(rmx["energy"] - r64["energy"]) / r64["energy"],
-                 "enstrophy_rel_err_vs_fp64": abs(rmx["enstrophy"] - r64["enstrophy"]) / r64["enstrophy"],
                  "passes": {k: {"avg_ms": v["avg_ms"], "GBs": v["GBs"], "frac": v["frac"]}
                             for k, v in rmx["passes"].items()}}
+        if a.err_steps:
+            print(f"bench: mixed vs fp64 over {a.err_steps} steps", file=sys.stderr, flush=True)
+            mixed["error_vs_fp64"] = mixed_error(dns_amd, n, nu, dt, u, a.err_steps)
     dom_name, dom = max(r64["passes"].items(), key=lambda kv: kv[1]["ms"])
     # HBM traffic per launch measured by rocprofv3 FETCH_SIZE/WRITE_SIZE runs
     # (calibrated on a copy kernel), committed with the profiles

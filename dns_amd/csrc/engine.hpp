@@ -70,9 +70,10 @@ struct dns_ctx {
   int w_ready;
   long long bytes;
   dnsk::Impl *impl;
-  // when set, step() copies the stage state U_{r+1} after stage r (r < 3)
-  // to rec[r] (used by the adjoint's forward recomputation)
-  double2 *rec[3];
+  // when set, step() records the physical velocity and vorticity of every
+  // stage of a one-step call: stage r into rec_phys + 6 r n^3, fields
+  // (u, v, w, omega_x, omega_y, omega_z) (the adjoint's recomputation, FP64)
+  double *rec_phys;
   void *adj;  // adjoint workspace (dnsk::AdjWork), allocated on first use
   int variant;            // launch shape (see Engine::step_launch)
   int use_graph;          // replay steps from a captured graph
@@ -245,6 +246,12 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
   // pencils across waves.
   static constexpr int PPW = z_pencils_per_block<N>();
   static constexpr int NF = N / 2 + 1;
+  // recording kernels exist for FP64 only (the adjoint's precision)
+  static constexpr bool REC_OK = sizeof(R) == 8;
+  // minimum waves per SIMD the z pass's registers must allow (n=1024 FP64:
+  // 2 instead of 1 spills 92-124 bytes per lane in stages 2-4; the z pass
+  // gains 10% but the step measured 1% slower, so 1)
+  static constexpr int MINW_Z = 1;
 
   size_t sr_size() const override { return sizeof(cx<SR>); }
   // bytes per member of the accumulator region: 3 double fields, or in
@@ -257,6 +264,12 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
   static const cx<double> *twD(dns_ctx *c) { return (const cx<double> *)c->tw64; }
 
   dim3 grid_x(int T, int m = 1) const { return dim3((NF + T - 1) / T, N, m); }
+  // grid of the x-pass kernels (k_xinv_curl, k_xfwd_rk_xinv): one-dimensional
+  // where x_tile pairs the tiles of a line
+  template <int T> dim3 grid_xp(int m = 1) const {
+    if constexpr (x_tiles_paired<T, R>()) return dim3((NF + T - 1) / T * N, 1, m);
+    else return grid_x(T, m);
+  }
 
   // Member m's slice of the W scratch, as bytes.
   static char *wslice(dns_ctx *c, int m) { return (char *)c->W + (size_t)m * c->g.mW * sizeof(cx<R>); }
@@ -350,7 +363,7 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
 
   int prepare(dns_ctx *c) override {
     if (c->w_ready) return DNS_OK;
-    const dim3 gx = grid_x(T_X, c->members), bx(T_X * TPP_X);
+    const dim3 gx = grid_xp<T_X>(c->members), bx(T_X * TPP_X);
     DNS_TIMED(c, 0, (k_xinv_curl<N, R, T_X, false, double, 1, E_X><<<gx, bx, 0, c->stream>>>(
                         (const double2 *)c->uhn, (cx<R> *)c->W, c->g, twR(c))));
     c->w_ready = 1;
@@ -379,7 +392,7 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
   int step(dns_ctx *c, long nsteps) override {
     int rc = prepare(c);
     if (rc) return rc;
-    if (c->use_graph && !c->timing && !c->capture && !c->rec[0]) return step_graph(c, nsteps);
+    if (c->use_graph && !c->timing && !c->capture && !c->rec_phys) return step_graph(c, nsteps);
     return step_launch(c, nsteps);
   }
 
@@ -448,7 +461,7 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
     static const double a_rk[4] = {1 / 6.0, 1 / 3.0, 1 / 3.0, 1 / 6.0};  // main.c:15
     static const double b_rk[4] = {0.5, 0.5, 1.0, 0.0};                 // main.c:16
     const dim3 gy = grid_x(TY, M), by(TY * TPP_Y);
-    const dim3 gx = grid_x(TX, M), bx(TX * TPP_X);
+    const dim3 gx = grid_xp<TX>(M), bx(TX * TPP_X);
     const dim3 gz(N * N / PP, 1, M), bz(PP * TPP);
     const dim3 gy_trunc((g.nk + TYF - 1) / TYF, N, M), byf(TYF * TPP_Y);  // dealiased kz columns only
     for (long st = 0; st < nsteps; st++) {
@@ -457,18 +470,27 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
         RKParams q{c->nudt, c->kmax, a_rk[rk], b_rk[rk]};
         const int sl = DNS_NPASS * rk;
         // pass A of this stage ran fused into the previous stage's E
+        // the adjoint's recording of this stage's u and omega (FP64, M = 1)
+        double *rp = (sizeof(R) == 8 && c->rec_phys) ? c->rec_phys + (size_t)rk * 6 * g.rs : nullptr;
         if (rk == 0) {
           DNS_TIMED(c, sl + 1, (k_ypass<N, R, TY, true, 1, PIPE_Y, false, E_Y, false, true><<<gy, by, 0, s>>>(W + 3 * g.wfs, 3, g, tw, g.mW)));
-          DNS_TIMED(c, sl + 2, (k_zfused<N, R, PP, 0, R><<<gz, bz, 0, s>>>(W, (R *)c->us, g, invn3, tw)));
+          if (rp)
+            DNS_TIMED(c, sl + 2, (k_zfused<N, R, PP, 0, R, MINW_Z, REC_OK><<<gz, bz, 0, s>>>(W, (R *)c->us, g, invn3, tw, rp)));
+          else
+            DNS_TIMED(c, sl + 2, (k_zfused<N, R, PP, 0, R, MINW_Z><<<gz, bz, 0, s>>>(W, (R *)c->us, g, invn3, tw)));
         } else {
           if constexpr (CX_IN_Y)
             DNS_TIMED(c, sl + 1, (k_ypass<N, R, TY, true, 1, PIPE_YC, false, E_Y, true, true><<<gy, by, 0, s>>>(W, 6, g, tw, g.mW)));
           else
             DNS_TIMED(c, sl + 1, (k_ypass<N, R, TY, true, 1, PIPE_Y, false, E_Y, false, true><<<gy, by, 0, s>>>(W, 6, g, tw, g.mW)));
-          if (rk < 3)
-            DNS_TIMED(c, sl + 2, (k_zfused<N, R, PP, 1, R><<<gz, bz, 0, s>>>(W, (R *)c->us, g, invn3, tw)));
+          if (rp && rk < 3)
+            DNS_TIMED(c, sl + 2, (k_zfused<N, R, PP, 1, R, MINW_Z, REC_OK><<<gz, bz, 0, s>>>(W, (R *)c->us, g, invn3, tw, rp)));
+          else if (rp)
+            DNS_TIMED(c, sl + 2, (k_zfused<N, R, PP, 2, R, MINW_Z, REC_OK><<<gz, bz, 0, s>>>(W, (R *)c->us, g, invn3, tw, rp)));
+          else if (rk < 3)
+            DNS_TIMED(c, sl + 2, (k_zfused<N, R, PP, 1, R, MINW_Z><<<gz, bz, 0, s>>>(W, (R *)c->us, g, invn3, tw)));
           else
-            DNS_TIMED(c, sl + 2, (k_zfused<N, R, PP, 2, R><<<gz, bz, 0, s>>>(W, (R *)c->us, g, invn3, tw)));
+            DNS_TIMED(c, sl + 2, (k_zfused<N, R, PP, 2, R, MINW_Z><<<gz, bz, 0, s>>>(W, (R *)c->us, g, invn3, tw)));
         }
         DNS_TIMED(c, sl + 3,
                   (k_ypass<N, R, TYF, false, 1, PIPE_YF, true, E_Y, false, true><<<gy_trunc, byf, 0, s>>>(W, 3, g, tw, g.mW)));
@@ -487,8 +509,6 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
             DNS_TIMED(c, sl + 5, (k_xfwd_rk_xinv<N, R, TX, 3, SR, 1, EPI_X3, E_X, !CX_IN_Y><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, ph, g, q, tw)));
             break;
         }
-        if (rk < 3 && c->rec[rk])
-          DNS_HIP(hipMemcpyAsync(c->rec[rk], uhr, 3 * sizeof(cx<SR>) * g.fs, hipMemcpyDeviceToDevice, s));
       }
       if (ph) c->have_phat = 1;
     }
@@ -662,31 +682,27 @@ int Engine<N, R, SR>::adj_step(dns_ctx *c, const double *uh, const double *us, c
     AdjWork *w;
     int rc = Adjoint<N>::workspace(c, w);
     if (rc) return rc;
+    if (w->recs.empty()) {
+      void *p = nullptr;
+      if ((rc = w->get(&p, w->REC))) return rc;
+      w->recs.push_back(p);
+    }
     hipStream_t s = c->stream;
     const size_t pitch = sizeof(double2) * c->nfp, width = sizeof(double2) * NF, rows = (size_t)3 * N * N;
-    // step-start state in w->Fw / w->lw (scratch until backward_step reuses
-    // them: it copies both into the context first); host spectra are
-    // x-major and go through tmpC / tmpN into the state layout
-    double2 *U = w->Fw;
-    double *u = w->lw;
-    DNS_HIP(hipMemset2DAsync(w->tmpC, pitch, 0, pitch, rows, s));
-    DNS_HIP(hipMemcpy2DAsync(w->tmpC, pitch, uh, width, width, rows, hipMemcpyDefault, s));
-    k_swap_rows<double2><<<N * N, 256, 0, s>>>(w->tmpC, U, N, c->g, 3);
+    // host spectra are x-major; tmpN takes them into the state layout
+    DNS_HIP(hipMemset2DAsync(w->tmpN, pitch, 0, pitch, rows, s));
+    DNS_HIP(hipMemcpy2DAsync(w->tmpN, pitch, uh, width, width, rows, hipMemcpyDefault, s));
+    k_swap_rows<double2><<<N * N, 256, 0, s>>>(w->tmpN, (double2 *)c->uhn, N, c->g, 3);
     DNS_LAUNCHED();
-    DNS_HIP(hipMemcpyAsync(u, us, w->R3, hipMemcpyDefault, s));
+    DNS_HIP(hipMemcpyAsync(c->us, us, w->R3, hipMemcpyDefault, s));
     DNS_HIP(hipMemset2DAsync(w->tmpN, pitch, 0, pitch, rows, s));
     DNS_HIP(hipMemcpy2DAsync(w->tmpN, pitch, lam_uh, width, width, rows, hipMemcpyDefault, s));
     k_swap_rows<double2><<<N * N, 256, 0, s>>>(w->tmpN, w->lamU, N, c->g, 3);
     DNS_LAUNCHED();
     DNS_HIP(hipMemcpyAsync(w->lamS, lam_us, w->R3, hipMemcpyDefault, s));
-    // backward_step overwrites Fw and lw; keep the inputs in the segment store
-    void *st = w->seg.empty() ? Adjoint<N>::state_alloc(c, w) : w->seg[0];
-    if (!st) return DNS_ENOMEM;
-    if (w->seg.empty()) w->seg.push_back(st);
-    DNS_HIP(hipMemcpyAsync(st, U, w->C3, hipMemcpyDeviceToDevice, s));
-    DNS_HIP(hipMemcpyAsync((char *)st + w->C3, u, w->R3, hipMemcpyDeviceToDevice, s));
-    rc = Adjoint<N>::backward_step(c, w, (const double2 *)st, (const double *)((char *)st + w->C3));
-    if (rc) return rc;
+    c->w_ready = 0;
+    if ((rc = Adjoint<N>::record(c, w->recs[0]))) return rc;
+    if ((rc = Adjoint<N>::backward_stages(c, w, (const double *)w->recs[0]))) return rc;
     k_swap_rows<double2><<<N * N, 256, 0, s>>>(w->lamU, w->tmpN, N, c->g, 3);
     DNS_LAUNCHED();
     DNS_HIP(hipMemcpy2DAsync(out_uh, width, w->tmpN, pitch, width, rows, hipMemcpyDefault, s));

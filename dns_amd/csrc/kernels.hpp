@@ -61,6 +61,27 @@ template <int N> constexpr int z_pencils_per_block() {
   return p;
 }
 
+// Column tile (bx) and row (by) of an x-pass block.  Tiles whose rows are
+// narrower than a 128-byte line (T complex elements) split each line between
+// two neighbouring tiles; such grids are launched one-dimensional and block
+// p takes the tile ordered so that the two tiles of a line run 8 blocks
+// apart, which the hardware dispatches to the same XCD (consecutive
+// workgroups go to the eight XCDs in turn): the second tile then finds the
+// line in that XCD's L2 instead of fetching it again.  ntx = column tiles
+// per row, nty = rows.
+template <int T, typename R> constexpr bool x_tiles_paired() { return T * sizeof(cx<R>) < 128; }
+template <int T, typename R> __device__ __forceinline__ void x_tile(int ntx, int nty, int &bx, int &by) {
+  if constexpr (x_tiles_paired<T, R>()) {
+    const long P = (long)ntx * nty, p = blockIdx.x;
+    const long l = p < P / 16 * 16 ? 2 * (8 * (p / 16) + p % 8) + ((p % 16) >= 8 ? 1 : 0) : p;
+    bx = (int)(l % ntx);
+    by = (int)(l / ntx);
+  } else {
+    bx = blockIdx.x;
+    by = blockIdx.y;
+  }
+}
+
 // Index i of an x or y axis lies inside the dealiased box.
 __device__ __forceinline__ bool in_box(int i, int n, int nk) { return i < nk || i > n - nk; }
 
@@ -104,9 +125,10 @@ __global__ __launch_bounds__(256) void k_swap_rows(const T *__restrict__ src, T 
 // MINW: minimum resident waves per SIMD the register allocation must allow.
 // PIPE: number of fields loaded ahead (0: none; 1: field f+1 is loaded
 // while field f is transformed; 2: fields f+1 and f+2).
-// TRUNC (forward transform of the nonlinear term only): the columns
-// k >= g.nk are not read and only rows inside the dealiased box are written;
-// the grid then covers ceil(nk/T) column tiles.
+// TRUNC: the grid covers ceil(nk/T) column tiles and columns k >= g.nk are
+// left alone.  Forward (the nonlinear term): only rows inside the dealiased
+// box are written.  Inverse (the adjoint's lamN): only rows inside the box
+// are read (the others are zero) and all rows are written.
 // CURLX (inverse transform of u, v, w and omega, nfields = 6): field 3 is not
 // read; curlX = I*(ky w - kz v) (main.c:361, same operation order) is formed
 // from fields 1 and 2 as loaded, then transformed and stored as field 3.
@@ -139,7 +161,8 @@ __global__ __launch_bounds__(T *(N / E), MINW) void k_ypass(cx<R> *__restrict__ 
 #pragma unroll
     for (int c = 0; c < E; c++) {
       const cx<R> *q = d + (long)(t + TPP * c) * g.nfp;
-      v[c] = ok ? (NTL ? ld_row<T>(q) : *q) : zero<R>();
+      const bool in = ok && (!(TRUNC && INV) || in_box(t + TPP * c, N, g.nk));
+      v[c] = in ? (NTL ? ld_row<T>(q) : *q) : zero<R>();
     }
   };
   auto fft_store = [&](cx<R>(&v)[E], int f) {
@@ -148,7 +171,7 @@ __global__ __launch_bounds__(T *(N / E), MINW) void k_ypass(cx<R> *__restrict__ 
       cx<R> *d = base + f * fstr;
 #pragma unroll
       for (int c = 0; c < E; c++)
-        if (!TRUNC || in_box(t + TPP * c, N, g.nk)) st<R>(d + (long)(t + TPP * c) * g.nfp, v[c]);
+        if (!TRUNC || INV || in_box(t + TPP * c, N, g.nk)) st<R>(d + (long)(t + TPP * c) * g.nfp, v[c]);
     }
   };
   cx<R> v[E], nx[E], nx2[E];
@@ -292,7 +315,9 @@ __global__ __launch_bounds__(T *(N / E), MINW) void k_xinv_curl(const cx<SI> *__
   __shared__ cx<R> twl[tw_size<N>()];
   tw = stage_twiddles<N, R>(twl, tw);
   const int p = threadIdx.x % T, t = threadIdx.x / T;
-  const int j = blockIdx.y, k = blockIdx.x * T + p;
+  int bx, j;
+  x_tile<T, R>((N / 2 + T) / T, N, bx, j);
+  const int k = bx * T + p;
   const bool ok = k < N / 2 + 1;
   uh += blockIdx.z * g.mS;
   W += blockIdx.z * g.mW;
@@ -335,20 +360,24 @@ template <typename R> __device__ __forceinline__ cx<R> lane_get(cx<R> v, int src
   }
 }
 template <int N, typename R>
-__device__ __forceinline__ void herm_load(HermRaw<N, R> &h, const cx<R> *X, const cx<R> *Y, int t) {
+__device__ __forceinline__ void herm_load(HermRaw<N, R> &h, const cx<R> *X, const cx<R> *Y, int t,
+                                          int lim = N / 2 + 1) {
   constexpr int E = Sched<N>::E, TPP = N / E, H = N / 2;
+  // Coefficients k >= lim are taken as zero and not read.
   if constexpr (TPP <= 64) {
     // Each stored element is loaded once (non-temporal): positions pos <= H
-    // directly; pos > H needs k = N - pos = TPP (E - c) - t, which thread
-    // TPP - t of the same pencil (same wave) holds at index E - 1 - c (for
-    // t = 0: its own index E - c, or the Nyquist element at c = E/2).
+    // directly (pos < H sit at c < E/2: t + TPP c < TPP E/2 = H); pos > H
+    // needs k = N - pos = TPP (E - c) - t, which thread TPP - t of the same
+    // pencil (same wave) holds at index E - 1 - c (for t = 0: its own index
+    // E - c, or the Nyquist element at c = E/2), passed between lanes.
 #pragma unroll
     for (int c = 0; c < E / 2; c++) {
-      h.x[c] = ldnt(X + t + TPP * c);
-      h.y[c] = Y ? ldnt(Y + t + TPP * c) : zero<R>();
+      const bool in = t + TPP * c < lim;
+      h.x[c] = in ? ldnt(X + t + TPP * c) : zero<R>();
+      h.y[c] = Y && in ? ldnt(Y + t + TPP * c) : zero<R>();
     }
     cx<R> nx = zero<R>(), ny = zero<R>();
-    if (t == 0) {
+    if (t == 0 && H < lim) {
       nx = ldnt(X + H);
       if (Y) ny = ldnt(Y + H);
     }
@@ -366,13 +395,16 @@ __device__ __forceinline__ void herm_load(HermRaw<N, R> &h, const cx<R> *X, cons
         h.y[c] = my;
       }
     }
-  } else {  // a pencil over several waves: each thread loads its mirrored positions too
+  } else {
+    // a pencil over several waves: each thread loads its mirrored positions
+    // too, with ordinary loads, so that the second fetch of an element finds
+    // it in L2 (the pencil's other half-wave loaded it)
 #pragma unroll
     for (int c = 0; c < E; c++) {
       const int pos = t + TPP * c;
       const int k = pos <= H ? pos : N - pos;
-      h.x[c] = X[k];
-      h.y[c] = Y ? Y[k] : zero<R>();
+      h.x[c] = k < lim ? X[k] : zero<R>();
+      h.y[c] = Y && k < lim ? Y[k] : zero<R>();
     }
   }
 }
@@ -389,9 +421,10 @@ __device__ __forceinline__ void herm_combine(cx<R> (&z)[Sched<N>::E], const Herm
   }
 }
 template <int N, typename R>
-__device__ __forceinline__ void herm_pair(cx<R> (&z)[Sched<N>::E], const cx<R> *X, const cx<R> *Y, int t) {
+__device__ __forceinline__ void herm_pair(cx<R> (&z)[Sched<N>::E], const cx<R> *X, const cx<R> *Y, int t,
+                                          int lim = N / 2 + 1) {
   HermRaw<N, R> h;
-  herm_load<N, R>(h, X, Y, t);
+  herm_load<N, R>(h, X, Y, t, lim);
   herm_combine<N, R>(z, h, t);
 }
 
@@ -424,10 +457,11 @@ __device__ __forceinline__ void split_store(const cx<R> (&f)[Sched<N>::E], cx<R>
 // (N_x, N_y) stored; N_z returned in nz.  Loads one Hermitian pair at a
 // time (register budget).  MODE 1, 2: h holds the raw loads of the row's
 // first pair (W fields 0, 1) on entry.
-template <int N, typename R, int MODE, typename UR>
+template <int N, typename R, int MODE, typename UR, bool REC = false>
 __device__ __forceinline__ void zrow(cx<R> *__restrict__ w, UR *__restrict__ usr, Geo g, R invn3,
                                      const cx<R> *__restrict__ tw, int t, cx<R> *lds, PencilSlot<N> slot,
-                                     const R (&oz0)[Sched<N>::E], R (&nz)[Sched<N>::E], const HermRaw<N, R> &h) {
+                                     const R (&oz0)[Sched<N>::E], R (&nz)[Sched<N>::E], const HermRaw<N, R> &h,
+                                     double *__restrict__ rec = nullptr) {
   constexpr int E = Sched<N>::E, TPP = N / E;
   R u[E], v[E], ww[E], ox[E], oy[E], oz[E];
   if constexpr (MODE == 0) {
@@ -474,6 +508,18 @@ __device__ __forceinline__ void zrow(cx<R> *__restrict__ w, UR *__restrict__ usr
       }
     }
   }
+  if constexpr (REC) {  // the adjoint's record of u and omega (physical)
+#pragma unroll
+    for (int c = 0; c < E; c++) {
+      const long l = t + TPP * c;
+      st_real<double>(rec + l, (double)u[c]);
+      st_real<double>(rec + g.rs + l, (double)v[c]);
+      st_real<double>(rec + 2 * g.rs + l, (double)ww[c]);
+      st_real<double>(rec + 3 * g.rs + l, (double)ox[c]);
+      st_real<double>(rec + 4 * g.rs + l, (double)oy[c]);
+      st_real<double>(rec + 5 * g.rs + l, (double)oz[c]);
+    }
+  }
   cx<R> f1[E];
 #pragma unroll
   for (int c = 0; c < E; c++) {
@@ -494,10 +540,13 @@ __device__ __forceinline__ void zrow(cx<R> *__restrict__ w, UR *__restrict__ usr
 // (N_x, N_y) are transformed and stored before N_z is transformed, so the
 // two forward transforms are never live together (9 VGPRs fewer in float
 // when measured).  Grid n*n/PPW blocks.
-template <int N, typename R, int PPW, int MODE, typename UR, int MINW = 1>
+// REC: u and omega of every point are also stored (double) into rec, six
+// physical fields [6][n][n][n] (the adjoint's forward recomputation).
+template <int N, typename R, int PPW, int MODE, typename UR, int MINW = 1, bool REC = false>
 __global__ __launch_bounds__(PPW *(N / Sched<N>::E), MINW) void k_zfused(cx<R> *__restrict__ W, UR *__restrict__ us,
                                                                     Geo g, R invn3,
-                                                                    const cx<R> *__restrict__ tw) {
+                                                                    const cx<R> *__restrict__ tw,
+                                                                    double *__restrict__ rec = nullptr) {
   constexpr int E = Sched<N>::E, TPP = N / E, H = N / 2;
   __shared__ cx<R> lds[PPW * padlen<N>()];
   const int p = threadIdx.x / TPP, t = threadIdx.x % TPP;
@@ -516,7 +565,8 @@ __global__ __launch_bounds__(PPW *(N / Sched<N>::E), MINW) void k_zfused(cx<R> *
   } else {
     herm_load<N, R>(h, w, w + g.wfs, t);
   }
-  zrow<N, R, MODE, UR>(w, us + row * N, g, invn3, tw, t, lds, slot, oz0, nz, h);
+  zrow<N, R, MODE, UR, REC>(w, us + row * N, g, invn3, tw, t, lds, slot, oz0, nz, h,
+                            REC ? rec + row * N : nullptr);
   // N_z is the transform of a real pencil: its first half is the r2c output.
   cx<R> f2[E];
 #pragma unroll
@@ -855,7 +905,9 @@ __global__ __launch_bounds__(T *(N / E), MINW) void k_xfwd_rk_xinv(cx<R> *__rest
   __shared__ cx<R> twl[tw_size<N>()];
   tw = stage_twiddles<N, R>(twl, tw);
   const int p = threadIdx.x % T, t = threadIdx.x / T;
-  const int j = blockIdx.y, k = blockIdx.x * T + p;
+  int bx, j;
+  x_tile<T, R>((N / 2 + T) / T, N, bx, j);
+  const int k = bx * T + p;
   const bool ok = k < N / 2 + 1;
   cx<R> U[E], V[E], Wv[E];
   forward_x_rk<N, R, T, STAGE, SR, EPI, E>(W, uhr, uhn, acc, phat, g, q, tw, t, p, j, k, ok, lds, U, V, Wv);

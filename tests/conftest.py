@@ -22,6 +22,22 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an AMD GPU (MI355X) and the built HIP library")
 
 
+@pytest.fixture(scope="session", autouse=True)
+def _torch_runtime_first(request):
+    """When GPU tests run, initialise torch's HIP runtime at the start of the
+    session, before libdns_hip's: the torch tests (autograd wrapper, device
+    buffers) then find their device however many library contexts came and
+    went before them (once, torch's late initialisation reported no HIP
+    device after the adjoint tests had run)."""
+    if any(item.get_closest_marker("gpu") for item in request.session.items):
+        try:
+            import torch
+        except ImportError:
+            return
+        if torch.cuda.is_available():
+            torch.cuda.init()
+
+
 def _build_oracle():
     if not (os.path.exists(ORACLE_LIB) and os.path.exists(ORACLE_CLI)):
         subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
