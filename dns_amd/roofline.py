@@ -139,3 +139,45 @@ def summarize(timing, n, precision, steps):
         d["ms_per_step"] = d["ms"] / steps
         d["bytes_per_launch"] = d["bytes"] / d["launches"]
     return per_pass
+
+
+def adjoint_launch_bytes(n):
+    """Algorithmic bytes of the fused backward stage's kernels (FP64), per
+    launch, keyed by role; a backward step launches xa once and yb, z, yd,
+    xe four times each (roles z.r3 / z.r12 / z.r0 and xe.first / xe.next /
+    xe.last by stage; yd moves six fields at stages 3..1, three at stage 0),
+    after one recorded forward step (record: the forward step's bytes plus
+    the 24 recorded physical fields).
+      xa   r lamU 3U;                       w lamV 3U, lamN partials 3Yc
+      yb   r 3Yc (box rows);                w 3Zc (all rows, kz < kmax)
+      z    r 3Zc + u, omega 6R (+ lamS 3R at stage 3);
+           w lam_omega, lam_u 6U (stage 0: 3U + lamS' 3R)
+      yd   r/w 6U (stage 0: 3U)
+      xe   r F 6U (stage 0: 3U), lamV 3U, lamUn 3U (+ lamU 3U between);
+           w lamUn 3U (+ lamV 3U, lamN partials 3Yc when fused with xa)"""
+    uc = n * n * (n // 2 + 1) * 16
+    rr = n ** 3 * 8
+    nk = n_box(n)
+    zc = n * n * nk * 16
+    yc = n * (2 * nk - 1) * nk * 16
+    return {
+        "xa": 3 * uc + 3 * uc + 3 * yc,
+        "yb": 3 * yc + 3 * zc,
+        "z.r3": 3 * zc + 6 * rr + 3 * rr + 6 * uc,
+        "z.r12": 3 * zc + 6 * rr + 6 * uc,
+        "z.r0": 3 * zc + 6 * rr + 3 * uc + 3 * rr,
+        "yd6": 12 * uc,
+        "yd3": 6 * uc,
+        "xe.first": 6 * uc + 3 * uc + 3 * uc + 3 * uc + 3 * uc + 3 * yc,
+        "xe.next": 6 * uc + 3 * uc + 3 * uc + 3 * uc + 3 * uc + 3 * uc + 3 * yc,
+        "xe.last": 3 * uc + 3 * uc + 3 * uc + 3 * uc,
+        "record": step_bytes_fused(n, "fp64") + 24 * rr,
+    }
+
+
+def adjoint_step_bytes(n):
+    """Bytes of one backward step (recorded forward step and four fused
+    backward stages)."""
+    b = adjoint_launch_bytes(n)
+    return (b["record"] + b["xa"] + 4 * b["yb"] + b["z.r3"] + 2 * b["z.r12"] + b["z.r0"] + 3 * b["yd6"] + b["yd3"]
+            + b["xe.first"] + 2 * b["xe.next"] + b["xe.last"])
