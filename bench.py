@@ -210,6 +210,8 @@ def adjoint(a):
             raw.tofile(os.path.join(tmp.name, name))
             jobs[(k, sgn)] = pool.submit(_oracle_final_enstrophy, tmp.name, name, nu, dt, nsteps, threads)
     s = dns_amd.Solver(n, nu, dt, "fp64")
+    if a.adj_direct:
+        s.set_kernel_timing(True)  # direct launches only (no graph capture)
     s.adjoint_forward(u0, 2, 1)  # workspace and graphs warm
     s.adjoint_backward()
     t0 = time.perf_counter()
@@ -442,6 +444,8 @@ def main():
                     help="steps of the FP64-vs-mixed error run (0: skip)")
     ap.add_argument("--adj-cpu-dirs", type=int, default=2,
                     help="directions also differenced with the CPU oracle")
+    ap.add_argument("--adj-direct", action="store_true",
+                    help="adjoint: launch every kernel directly, no graph capture (for tracing profilers)")
     ap.add_argument("--dump", action="store_true", help="wall time of dns -d against dns instead")
     ap.add_argument("--dump-dir", default="/tmp", help="directory the snapshots are written to")
     a = ap.parse_args()
