@@ -77,12 +77,13 @@ def gpu_run(dns_amd, n, nu, dt, precision, u, warmup, steps):
 
 
 def mixed_error(dns_amd, n, nu, dt, u, steps, every=10):
-    """Largest relative deviation of the mixed run's energy and enstrophy
-    from the FP64 run over every diagnostics line (every 10 steps, as dns
-    prints them) of a `steps`-step run, and the max-norm deviation of the
-    final physical velocity relative to max |u|."""
+    """Largest relative deviation of the energy and enstrophy of each mixed
+    form (stage increments summed in float: "mixed"; in double:
+    "mixed_acc64") from the FP64 run over every diagnostics line (every 10
+    steps, as dns prints them) of a `steps`-step run, and the max-norm
+    deviation of the final physical velocity relative to max |u|."""
     hist, fin = {}, {}
-    for prec in ("fp64", "mixed"):
+    for prec in ("fp64", "mixed", "mixed_acc64"):
         with dns_amd.Solver(n, nu, dt, prec) as s:
             s.load_velocity(u)
             lines = [s.diagnostics()]
@@ -91,11 +92,16 @@ def mixed_error(dns_amd, n, nu, dt, u, steps, every=10):
                 lines.append(s.diagnostics())
             hist[prec] = np.array(lines)
             fin[prec] = s.get_stale_velocity()
-    a, b = hist["fp64"], hist["mixed"]
-    return {"steps": steps, "lines": len(a),
-            "max_rel_energy": float(np.max(np.abs(b[:, 0] - a[:, 0]) / a[:, 0])),
-            "max_rel_enstrophy": float(np.max(np.abs(b[:, 1] - a[:, 1]) / a[:, 1])),
-            "final_velocity_max_norm": float(np.max(np.abs(fin["mixed"] - fin["fp64"])) / np.max(np.abs(fin["fp64"])))}
+    a = hist["fp64"]
+
+    def dev(p):
+        b = hist[p]
+        return {"max_rel_energy": float(np.max(np.abs(b[:, 0] - a[:, 0]) / a[:, 0])),
+                "max_rel_enstrophy": float(np.max(np.abs(b[:, 1] - a[:, 1]) / a[:, 1])),
+                "final_velocity_max_norm": float(np.max(np.abs(fin[p] - fin["fp64"])) / np.max(np.abs(fin["fp64"])))}
+    out = {"steps": steps, "lines": len(a), **dev("mixed")}
+    out["mixed_acc64"] = dev("mixed_acc64")
+    return out
 
 
 def cpu_model():
@@ -482,9 +488,17 @@ def main():
                  "speedup_vs_fp64": rmx["steps_per_s"] / r64["steps_per_s"],
                  "passes": {k: {"avg_ms": v["avg_ms"], "GBs": v["GBs"], "frac": v["frac"]}
                             for k, v in rmx["passes"].items()}}
+        # the same form with the stage increments summed in double
+        print("bench: mixed_acc64", file=sys.stderr, flush=True)
+        r2 = gpu_run(dns_amd, n, nu, dt, "mixed_acc64", u, a.warmup, a.steps)
+        mixed["acc64"] = {"steps_per_s": r2["steps_per_s"], "ms_per_step": r2["ms_per_step"],
+                          "step_bytes": roofline.step_bytes_fused(n, "mixed_acc64"),
+                          "speedup_vs_fp64": r2["steps_per_s"] / r64["steps_per_s"]}
         if a.err_steps:
-            print(f"bench: mixed vs fp64 over {a.err_steps} steps", file=sys.stderr, flush=True)
-            mixed["error_vs_fp64"] = mixed_error(dns_amd, n, nu, dt, u, a.err_steps)
+            print(f"bench: mixed forms vs fp64 over {a.err_steps} steps", file=sys.stderr, flush=True)
+            err = mixed_error(dns_amd, n, nu, dt, u, a.err_steps)
+            mixed["acc64"]["error_vs_fp64"] = err.pop("mixed_acc64")
+            mixed["error_vs_fp64"] = err
     dom_name, dom = max(r64["passes"].items(), key=lambda kv: kv[1]["ms"])
     # HBM traffic per launch measured by rocprofv3 FETCH_SIZE/WRITE_SIZE runs
     # (calibrated on a copy kernel), committed with the profiles

@@ -19,7 +19,8 @@ ROOT = os.path.dirname(PKG)
 # DNS_LIB selects another build of the library (e.g. to compare two builds)
 LIB_PATH = os.environ.get("DNS_LIB") or os.path.join(PKG, "libdns_hip.so")
 HEADER = os.path.join(ROOT, "include", "dns.h")
-FP64, MIXED = 0, 1
+FP64, MIXED, MIXED_ACC64 = 0, 1, 2
+PRECISIONS = {"fp64": FP64, "mixed": MIXED, "mixed_acc64": MIXED_ACC64}
 KERNEL_SLOTS = 24
 
 DNS_OK, DNS_EINVAL, DNS_ENODEV, DNS_ENOMEM, DNS_EHIP, DNS_ENONFINITE = 0, -1, -2, -3, -4, -5
@@ -116,7 +117,7 @@ class Solver:
 
     def __init__(self, n, nu, dt, precision=FP64, device=0):
         if isinstance(precision, str):
-            precision = {"fp64": FP64, "mixed": MIXED}[precision]
+            precision = PRECISIONS[precision]
         self.n, self.nu, self.dt, self.precision = int(n), float(nu), float(dt), precision
         self._ctx = ctypes.c_void_p()
         _check(lib().dns_create(ctypes.byref(self._ctx), self.n, self.nu, self.dt, precision, device),
@@ -269,7 +270,7 @@ class Batch:
 
     def __init__(self, ns, nus, dts, precision=FP64, device=0):
         if isinstance(precision, str):
-            precision = {"fp64": FP64, "mixed": MIXED}[precision]
+            precision = PRECISIONS[precision]
         self.ns = [int(v) for v in ns]
         m = len(self.ns)
         n_arr = np.ascontiguousarray(self.ns, dtype=np.int32)

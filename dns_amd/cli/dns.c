@@ -6,7 +6,7 @@
  *   dns [-v] [-d] -i <input.raw> -n <viscosity> -t <end time> -s <time step>
  *
  * Environment (opt-in, defaults keep the reference behaviour):
- *   DNS_PRECISION=fp64|mixed   arithmetic of the step (default fp64)
+ *   DNS_PRECISION=fp64|mixed|mixed_acc64   arithmetic of the step (default fp64)
  *   DNS_DEVICE=<ordinal>       HIP device (default 0)
  *
  * Differences that cannot be avoided: under -v the reference prints
@@ -304,8 +304,10 @@ int main(int argc, char **argv) {
   const char *env = getenv("DNS_PRECISION");
   if (env && strcmp(env, "mixed") == 0)
     precision = DNS_MIXED;
+  else if (env && strcmp(env, "mixed_acc64") == 0)
+    precision = DNS_MIXED_ACC64;
   else if (env && *env && strcmp(env, "fp64") != 0) {
-    fprintf(stderr, "dns: error: DNS_PRECISION must be fp64 or mixed\n");
+    fprintf(stderr, "dns: error: DNS_PRECISION must be fp64, mixed or mixed_acc64\n");
     exit(1);
   }
   if ((env = getenv("DNS_DEVICE")) != NULL && *env)

@@ -52,7 +52,7 @@ static int ctx_create(dns_ctx **out, int n, int members, const double *nu, const
                       int device) {
   *out = nullptr;
   g_err[0] = 0;
-  if (precision != DNS_FP64 && precision != DNS_MIXED) return DNS_EINVAL;
+  if (precision != DNS_FP64 && precision != DNS_MIXED && precision != DNS_MIXED_ACC64) return DNS_EINVAL;
   if (members < 1) return DNS_EINVAL;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {

@@ -150,13 +150,13 @@ inline hipEvent_t ev_get(dns_ctx *c) {
 inline void t_begin(dns_ctx *c, int slot) {
   if (!c->timing) return;
   dns_ctx::TimedLaunch t{slot, ev_get(c), ev_get(c)};
-  if (t.a) hipEventRecord(t.a, c->stream);
+  if (t.a) (void)hipEventRecord(t.a, c->stream);
   c->ev_pending.push_back(t);
 }
 inline void t_end(dns_ctx *c) {
   if (!c->timing || c->ev_pending.empty()) return;
   hipEvent_t b = c->ev_pending.back().b;
-  if (b) hipEventRecord(b, c->stream);
+  if (b) (void)hipEventRecord(b, c->stream);
 }
 #define DNS_TIMED(c, slot, launch) \
   do {                             \
@@ -196,7 +196,7 @@ template <int N> std::vector<long double> twiddles_ld() {
 #include "adjoint_engine.hpp"
 namespace dnsk {
 
-template <int N, typename R, typename SR> struct Engine final : Impl {
+template <int N, typename R, typename SR, bool S64 = false> struct Engine final : Impl {
   static constexpr int E = Sched<N>::E, TPP = N / E;
   // Column tiles of the strided passes: 128-byte row segments (8 double /
   // 16 float columns).  Blocks holding double state or the fused x pass
@@ -255,9 +255,12 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
 
   size_t sr_size() const override { return sizeof(cx<SR>); }
   // bytes per member of the accumulator region: 3 double fields, or in
-  // mixed precision 6 float fields (float copy of Un and the running sum of
-  // the increments)
-  size_t acc_bytes(long fs) const override { return sizeof(SR) == 4 ? 6 * sizeof(float2) * fs : 3 * sizeof(double2) * fs; }
+  // mixed precision the float copy of Un (3 float fields) and the running
+  // sum of the increments (3 float fields, or 3 double fields with S64)
+  size_t acc_bytes(long fs) const override {
+    return sizeof(SR) == 4 ? (3 * sizeof(float2) + 3 * (S64 ? sizeof(double2) : sizeof(float2))) * fs
+                           : 3 * sizeof(double2) * fs;
+  }
   size_t r_size() const override { return sizeof(R); }
 
   static const cx<R> *twR(dns_ctx *c) { return (const cx<R> *)(sizeof(R) == 8 ? c->tw64 : c->tw32); }
@@ -295,7 +298,7 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
       const long m0 = m < 0 ? 0 : m, m1 = m < 0 ? c->members : m + 1;
       for (long k = m0; k < m1; k++) {
         const double *un = (const double *)((double2 *)c->uhn + k * c->g.mS);
-        float *cp = (float *)((float2 *)c->acc + k * 2 * c->g.mS);
+        float *cp = (float *)((float2 *)c->acc + k * (S64 ? 3 : 2) * c->g.mS);
         k_convert<double, float><<<4096, 256, 0, c->stream>>>(un, cp, 2 * c->g.mS);
         DNS_LAUNCHED();
       }
@@ -497,16 +500,16 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
         // E fused with the next stage's A (the next step's stage 1 after rk 3)
         switch (rk) {
           case 0:
-            DNS_TIMED(c, sl + 5, (k_xfwd_rk_xinv<N, R, TX, 0, SR, 1, EPI_X, E_X, !CX_IN_Y><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, nullptr, g, q, tw)));
+            DNS_TIMED(c, sl + 5, (k_xfwd_rk_xinv<N, R, TX, 0, SR, 1, EPI_X, E_X, !CX_IN_Y, S64><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, nullptr, g, q, tw)));
             break;
           case 1:
-            DNS_TIMED(c, sl + 5, (k_xfwd_rk_xinv<N, R, TX, 1, SR, 1, EPI_X, E_X, !CX_IN_Y><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, nullptr, g, q, tw)));
+            DNS_TIMED(c, sl + 5, (k_xfwd_rk_xinv<N, R, TX, 1, SR, 1, EPI_X, E_X, !CX_IN_Y, S64><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, nullptr, g, q, tw)));
             break;
           case 2:
-            DNS_TIMED(c, sl + 5, (k_xfwd_rk_xinv<N, R, TX, 2, SR, 1, EPI_X, E_X, !CX_IN_Y><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, nullptr, g, q, tw)));
+            DNS_TIMED(c, sl + 5, (k_xfwd_rk_xinv<N, R, TX, 2, SR, 1, EPI_X, E_X, !CX_IN_Y, S64><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, nullptr, g, q, tw)));
             break;
           default:
-            DNS_TIMED(c, sl + 5, (k_xfwd_rk_xinv<N, R, TX, 3, SR, 1, EPI_X3, E_X, !CX_IN_Y><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, ph, g, q, tw)));
+            DNS_TIMED(c, sl + 5, (k_xfwd_rk_xinv<N, R, TX, 3, SR, 1, EPI_X3, E_X, !CX_IN_Y, S64><<<gx, bx, 0, s>>>(W, uhr, uhn, acc, ph, g, q, tw)));
             break;
         }
       }
@@ -656,8 +659,8 @@ template <int N, typename R, typename SR> struct Engine final : Impl {
   }
 };
 
-template <int N, typename R, typename SR>
-int Engine<N, R, SR>::adj_forward(dns_ctx *c, const double *u0, long nsteps, long every) {
+template <int N, typename R, typename SR, bool S64>
+int Engine<N, R, SR, S64>::adj_forward(dns_ctx *c, const double *u0, long nsteps, long every) {
   if constexpr (sizeof(R) != 8) {
     return DNS_EINVAL;
   } else {
@@ -665,7 +668,8 @@ int Engine<N, R, SR>::adj_forward(dns_ctx *c, const double *u0, long nsteps, lon
   }
 }
 
-template <int N, typename R, typename SR> int Engine<N, R, SR>::adj_backward(dns_ctx *c, double seed, double *grad) {
+template <int N, typename R, typename SR, bool S64>
+int Engine<N, R, SR, S64>::adj_backward(dns_ctx *c, double seed, double *grad) {
   if constexpr (sizeof(R) != 8) {
     return DNS_EINVAL;
   } else {
@@ -673,9 +677,9 @@ template <int N, typename R, typename SR> int Engine<N, R, SR>::adj_backward(dns
   }
 }
 
-template <int N, typename R, typename SR>
-int Engine<N, R, SR>::adj_step(dns_ctx *c, const double *uh, const double *us, const double *lam_uh,
-                               const double *lam_us, double *out_uh, double *out_us) {
+template <int N, typename R, typename SR, bool S64>
+int Engine<N, R, SR, S64>::adj_step(dns_ctx *c, const double *uh, const double *us, const double *lam_uh,
+                                    const double *lam_us, double *out_uh, double *out_us) {
   if constexpr (sizeof(R) != 8) {
     return DNS_EINVAL;
   } else {
@@ -715,6 +719,7 @@ int Engine<N, R, SR>::adj_step(dns_ctx *c, const double *uh, const double *us, c
 
 template <int N> Impl *make_engine(int prec) {
   if (prec == DNS_MIXED) return new Engine<N, float, float>();
+  if (prec == DNS_MIXED_ACC64) return new Engine<N, float, float, true>();
   return new Engine<N, double, double>();
 }
 

@@ -657,6 +657,8 @@ __global__ __launch_bounds__(PPW *(N / Sched<N>::E)) void k_zc2r(const cx<R> *__
 // accurate anyway, being made from the float nonlinear term); stage 3 forms
 // (Un + S) + a3 dU3 in double and also writes its float copy, which stages
 // 0..2 of the next step read (whoever sets Un from outside refreshes it).
+// S64 (mixed state only) keeps S in double instead: the increments are
+// float-accurate, their sum is formed and stored in double.
 // The epilogue runs in double for the FP64 state; for the mixed state dU
 // and P are evaluated in float (their inputs are float) and only the stage
 // 3 sum Un + S + a3 dU is formed in double.  phat (optional, stage 3)
@@ -667,7 +669,7 @@ __global__ __launch_bounds__(PPW *(N / Sched<N>::E)) void k_zc2r(const cx<R> *__
 // EPI selects the epilogue's batching: 0 = default (below), otherwise
 // BC = EPI & 63 positions per batch and prefetch of the next batch if
 // EPI & 64.  E = elements per thread (a multiple of every radix of N).
-template <int N, typename R, int T, int STAGE, typename SR, int EPI = 0, int E = Sched<N>::E>
+template <int N, typename R, int T, int STAGE, typename SR, int EPI = 0, int E = Sched<N>::E, bool S64 = false>
 __device__ __forceinline__ void forward_x_rk(const cx<R> *__restrict__ W, cx<SR> *__restrict__ uhr,
                                              double2 *__restrict__ uhn, double2 *__restrict__ acc,
                                              double2 *__restrict__ phat, Geo g, RKParams q,
@@ -689,7 +691,9 @@ __device__ __forceinline__ void forward_x_rk(const cx<R> *__restrict__ W, cx<SR>
   W += blockIdx.z * g.mW;
   uhr += blockIdx.z * g.mS;
   uhn += blockIdx.z * g.mS;
-  float2 *inc = (float2 *)acc + blockIdx.z * 2 * g.mS;  // mixed: [Un | S]
+  // mixed: [float Un | S], S float or (S64) double
+  float2 *inc = (float2 *)acc + blockIdx.z * (S64 ? 3 : 2) * g.mS;
+  double2 *s64 = (double2 *)(inc + 3 * g.fs);
   acc += blockIdx.z * g.mS;
   if (phat) phat += blockIdx.z * g.fs;
   const double2 nd = q.nudt[blockIdx.z];
@@ -731,7 +735,7 @@ __device__ __forceinline__ void forward_x_rk(const cx<R> *__restrict__ W, cx<SR>
     double2 un[3][BC];  // FP64 stages 1,2 and mixed stage 3: Un
     float2 unf[3][BC];  // mixed stages 1,2: float copy of Un
     double2 ac[3][BC];  // FP64 stages 1..3: accumulator
-    float2 sf[3][BC];   // mixed stages 1..3: running sum S
+    std::conditional_t<S64, double2, float2> sf[3][BC];  // mixed stages 1..3: running sum S
   };
   constexpr int NB = E / BC;
   auto load = [&](Batch &B, auto BB) {
@@ -748,7 +752,8 @@ __device__ __forceinline__ void forward_x_rk(const cx<R> *__restrict__ W, cx<SR>
         if constexpr (MIXED_STATE) {
           if constexpr (STAGE == 1 || STAGE == 2) B.unf[f][i] = ld_row<T>(inc + lf);
           if constexpr (STAGE == 3) B.un[f][i] = ld_row<T>(uhn + lf);
-          if constexpr (STAGE > 0) B.sf[f][i] = ld_row<T>(inc + 3 * g.fs + lf);
+          if constexpr (STAGE > 0 && S64) B.sf[f][i] = ld_row<T>(s64 + lf);
+          else if constexpr (STAGE > 0) B.sf[f][i] = ld_row<T>(inc + 3 * g.fs + lf);
         } else {
           if constexpr (STAGE == 1 || STAGE == 2) B.un[f][i] = ld_row<T>(uhn + lf);
           if constexpr (STAGE > 0) B.ac[f][i] = ld_row<T>(acc + lf);
@@ -797,7 +802,7 @@ __device__ __forceinline__ void forward_x_rk(const cx<R> *__restrict__ W, cx<SR>
           const float2 du = e[f];
           if constexpr (STAGE == 3) {
             const double2 un = B.un[f][i];
-            const float2 sa = B.sf[f][i];
+            const auto sa = B.sf[f][i];
             double2 nn = make_double2(un.x + (double)sa.x, un.y + (double)sa.y);
             nn = make_double2(nn.x + q.a * (double)du.x, nn.y + q.a * (double)du.y);
             st<double>(uhn + lf, nn);
@@ -808,79 +813,67 @@ __device__ __forceinline__ void forward_x_rk(const cx<R> *__restrict__ W, cx<SR>
           const float2 un = STAGE == 0 ? ur : B.unf[f][i];
           const float2 nr = make_float2(un.x + bf * du.x, un.y + bf * du.y);
           st<SR>(uhr + lf, cvt<SR, float>(nr));
-          const float2 sa = STAGE == 0 ? make_float2(0.f, 0.f) : B.sf[f][i];
-          st<float>(inc + 3 * g.fs + lf,
-                    STAGE == 0 ? make_float2(af * du.x, af * du.y) : make_float2(sa.x + af * du.x, sa.y + af * du.y));
+          if constexpr (S64) {
+            const double2 ad = make_double2(q.a * (double)du.x, q.a * (double)du.y);
+            if constexpr (STAGE == 0) {
+              st<double>(s64 + lf, ad);
+            } else {
+              const double2 sa = B.sf[f][i];
+              st<double>(s64 + lf, make_double2(sa.x + ad.x, sa.y + ad.y));
+            }
+          } else {
+            const float2 sa = STAGE == 0 ? make_float2(0.f, 0.f) : B.sf[f][i];
+            st<float>(inc + 3 * g.fs + lf,
+                      STAGE == 0 ? make_float2(af * du.x, af * du.y) : make_float2(sa.x + af * du.x, sa.y + af * du.y));
+          }
           nxt[f] = cvt<R, float>(nr);
         }
         U[c] = nxt[0];
         V[c] = nxt[1];
         Wv[c] = nxt[2];
-        continue;
-      }
-      double2 d[3];
+      } else {
+        double2 d[3];
 #pragma unroll
-      for (int f = 0; f < 3; f++) d[f] = make_double2((double)F[f][c].x * fac, (double)F[f][c].y * fac);
-      double2 P = make_double2(0.0, 0.0);
-      if (kk > 0) {
-        P.x = (d[0].x * kx + d[1].x * ky + d[2].x * kz) / kk;
-        P.y = (d[0].y * kx + d[1].y * ky + d[2].y * kz) / kk;
-      }
-      if (phat) st<double>(phat + l, P);
-      const double vk = nudt * kk;
-      const double kv[3] = {kx, ky, kz};
-      cx<R> nxt[3];
+        for (int f = 0; f < 3; f++) d[f] = make_double2((double)F[f][c].x * fac, (double)F[f][c].y * fac);
+        double2 P = make_double2(0.0, 0.0);
+        if (kk > 0) {
+          P.x = (d[0].x * kx + d[1].x * ky + d[2].x * kz) / kk;
+          P.y = (d[0].y * kx + d[1].y * ky + d[2].y * kz) / kk;
+        }
+        if (phat) st<double>(phat + l, P);
+        const double vk = nudt * kk;
+        const double kv[3] = {kx, ky, kz};
+        cx<R> nxt[3];
 #pragma unroll
-      for (int f = 0; f < 3; f++) {
-        const long lf = f * g.fs + l;
-        const double2 ur = cvt<double, URS>(B.ur[f][i]);
-        d[f].x -= P.x * kv[f] + vk * ur.x;
-        d[f].y -= P.y * kv[f] + vk * ur.y;
-        const double2 du = d[f];
-        if constexpr (MIXED_STATE) {
+        for (int f = 0; f < 3; f++) {
+          const long lf = f * g.fs + l;
+          const double2 ur = cvt<double, URS>(B.ur[f][i]);
+          d[f].x -= P.x * kv[f] + vk * ur.x;
+          d[f].y -= P.y * kv[f] + vk * ur.y;
+          const double2 du = d[f];
           if constexpr (STAGE == 0) {
             const cx<SR> nr = cvt<SR, double>(make_double2(ur.x + q.b * du.x, ur.y + q.b * du.y));
             st<SR>(uhr + lf, nr);
-            st<float>(inc + 3 * g.fs + lf, cvt<float, double>(make_double2(q.a * du.x, q.a * du.y)));
+            st<double>(acc + lf, make_double2(ur.x + q.a * du.x, ur.y + q.a * du.y));
             nxt[f] = cvt<R, SR>(nr);
           } else if constexpr (STAGE < 3) {
-            const double2 un = cvt<double, float>(B.unf[f][i]);
-            const double2 sa = cvt<double, float>(B.sf[f][i]);
+            const double2 un = B.un[f][i];
             const cx<SR> nr = cvt<SR, double>(make_double2(un.x + q.b * du.x, un.y + q.b * du.y));
             st<SR>(uhr + lf, nr);
-            st<float>(inc + 3 * g.fs + lf, cvt<float, double>(make_double2(sa.x + q.a * du.x, sa.y + q.a * du.y)));
+            const double2 ac = B.ac[f][i];
+            st<double>(acc + lf, make_double2(ac.x + q.a * du.x, ac.y + q.a * du.y));
             nxt[f] = cvt<R, SR>(nr);
           } else {
-            const double2 un = B.un[f][i];
-            const double2 sa = cvt<double, float>(B.sf[f][i]);
-            double2 nn = make_double2(un.x + sa.x, un.y + sa.y);
-            nn = make_double2(nn.x + q.a * du.x, nn.y + q.a * du.y);
+            const double2 ac = B.ac[f][i];
+            const double2 nn = make_double2(ac.x + q.a * du.x, ac.y + q.a * du.y);
             st<double>(uhn + lf, nn);
-            st<float>(inc + lf, cvt<float, double>(nn));
             nxt[f] = cvt<R, double>(nn);
           }
-        } else if constexpr (STAGE == 0) {
-          const cx<SR> nr = cvt<SR, double>(make_double2(ur.x + q.b * du.x, ur.y + q.b * du.y));
-          st<SR>(uhr + lf, nr);
-          st<double>(acc + lf, make_double2(ur.x + q.a * du.x, ur.y + q.a * du.y));
-          nxt[f] = cvt<R, SR>(nr);
-        } else if constexpr (STAGE < 3) {
-          const double2 un = B.un[f][i];
-          const cx<SR> nr = cvt<SR, double>(make_double2(un.x + q.b * du.x, un.y + q.b * du.y));
-          st<SR>(uhr + lf, nr);
-          const double2 ac = B.ac[f][i];
-          st<double>(acc + lf, make_double2(ac.x + q.a * du.x, ac.y + q.a * du.y));
-          nxt[f] = cvt<R, SR>(nr);
-        } else {
-          const double2 ac = B.ac[f][i];
-          const double2 nn = make_double2(ac.x + q.a * du.x, ac.y + q.a * du.y);
-          st<double>(uhn + lf, nn);
-          nxt[f] = cvt<R, double>(nn);
         }
+        U[c] = nxt[0];
+        V[c] = nxt[1];
+        Wv[c] = nxt[2];
       }
-      U[c] = nxt[0];
-      V[c] = nxt[1];
-      Wv[c] = nxt[2];
     }
   };
   if constexpr (PF) {
@@ -908,7 +901,7 @@ __device__ __forceinline__ void forward_x_rk(const cx<R> *__restrict__ W, cx<SR>
 // the re-read of the state.  W fields 0..2 (this stage's N) are read before
 // the same pencils' fields 0..5 are overwritten with the next partials.
 template <int N, typename R, int T, int STAGE, typename SR, int MINW = 1, int EPI = 0, int E = Sched<N>::E,
-          bool CX = false>
+          bool CX = false, bool S64 = false>
 __global__ __launch_bounds__(T *(N / E), MINW) void k_xfwd_rk_xinv(cx<R> *__restrict__ W,
                                                                        cx<SR> *__restrict__ uhr,
                                                                        double2 *__restrict__ uhn,
@@ -924,7 +917,7 @@ __global__ __launch_bounds__(T *(N / E), MINW) void k_xfwd_rk_xinv(cx<R> *__rest
   const int k = bx * T + p;
   const bool ok = k < N / 2 + 1;
   cx<R> U[E], V[E], Wv[E];
-  forward_x_rk<N, R, T, STAGE, SR, EPI, E>(W, uhr, uhn, acc, phat, g, q, tw, t, p, j, k, ok, lds, U, V, Wv);
+  forward_x_rk<N, R, T, STAGE, SR, EPI, E, S64>(W, uhr, uhn, acc, phat, g, q, tw, t, p, j, k, ok, lds, U, V, Wv);
   curl_inverse_x<N, R, T, (STAGE < 3), E, CX>(U, V, Wv, W + blockIdx.z * g.mW, g, t, p, j, k, ok, lds, tw);
 }
 

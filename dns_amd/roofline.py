@@ -92,14 +92,15 @@ def slot_bytes(n, precision="fp64"):
                 e = 3 * yc * cr + 3 * uc * csr + 6 * uc * c64 + 3 * uc * csr + 3 * uc * c64
             else:
                 e = 3 * yc * cr + 3 * uc * csr + 3 * uc * c64 + 3 * uc * c64
-        else:  # float copy of Un and float running sum S; Un, Un+1 double
+        else:  # float copy of Un and running sum S (float; double in mixed_acc64); Un, Un+1 double
             c32 = 8
+            cs = c64 if precision == "mixed_acc64" else c32
             if st == 0:    # r Un copy; w Ur(SR), S
-                e = 3 * yc * cr + 3 * uc * c32 + 3 * uc * csr + 3 * uc * c32
+                e = 3 * yc * cr + 3 * uc * c32 + 3 * uc * csr + 3 * uc * cs
             elif st < 3:   # r Ur, Un copy, S; w Ur, S
-                e = 3 * yc * cr + 3 * uc * csr + 6 * uc * c32 + 3 * uc * csr + 3 * uc * c32
+                e = 3 * yc * cr + 3 * uc * csr + 3 * uc * c32 + 3 * uc * cs + 3 * uc * csr + 3 * uc * cs
             else:          # r Ur, Un(f64), S; w Un+1(f64) and its float copy
-                e = 3 * yc * cr + 3 * uc * csr + 3 * uc * c64 + 3 * uc * c32 + 3 * uc * c64 + 3 * uc * c32
+                e = 3 * yc * cr + 3 * uc * csr + 3 * uc * c64 + 3 * uc * cs + 3 * uc * c64 + 3 * uc * c32
         # E fused with the next stage's A: A's writes without its read
         nxt_writes = 3 * uc * cr if st == 3 else ax * uc * cr
         for p, v in zip(PASSES, (a, b, c, d, e, e + nxt_writes)):

@@ -48,8 +48,12 @@ enum {
  *            with twiddles generated in double; the stage increments are
  *            summed in float (S), and the step-start spectral state U_n,
  *            the new state U_n+1 = (U_n + S) + a_4 dU_4, and the
- *            energy/enstrophy reductions are double. */
-enum dns_precision { DNS_FP64 = 0, DNS_MIXED = 1 };
+ *            energy/enstrophy reductions are double.
+ * DNS_MIXED_ACC64: as DNS_MIXED, but the running sum S of the stage
+ *            increments is formed and stored in double (FP64 RK4
+ *            accumulation; 3 more float fields of device memory and 10.5%
+ *            more bytes per step at n=512). */
+enum dns_precision { DNS_FP64 = 0, DNS_MIXED = 1, DNS_MIXED_ACC64 = 2 };
 
 typedef struct dns_ctx dns_ctx;
 

@@ -15,7 +15,7 @@ def _ic(n, m):
     return tgv_field(n)[:3] if m % 2 == 0 else random_field(n, seed=100 + m)
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "mixed_acc64"])
 def test_members_match_single_runs(precision):
     ns = [32, 48, 32, 64, 48, 32]
     nus = [0.01, 0.02, 1.0 / 400, 0.005, 0.0125, 1.0 / 1600]

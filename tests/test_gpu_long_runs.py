@@ -10,7 +10,14 @@
 * The ensemble's radix-3 grids, n=96 and n=192 (Taylor-Green and isotropic,
   Re 1600, dt=1e-3, 1000 steps), to the same bounds.
 * Taylor-Green at n=512, nu=1/3000, dt=5e-4, 500 steps: the same bounds on
-  every tenth line."""
+  every tenth line.
+Every run is made in both mixed forms: DNS_MIXED (stage increments summed
+in float) and DNS_MIXED_ACC64 (summed in double); both meet the bounds.
+The float sum's own error, the deviation of the DNS_MIXED run from the
+DNS_MIXED_ACC64 run, is bounded by 1e-6 (E, Z) and 1e-4 (u), and by the
+deviation of the DNS_MIXED run from FP64: the float transforms, not the
+float sum, dominate the error (measured: 3-10x smaller, at most 2.3e-8 in
+E, 4.1e-8 in Z, 1.5e-5 in u)."""
 import subprocess
 
 import numpy as np
@@ -39,14 +46,27 @@ def _run(n, nu, dt, steps, u, precision, every=10):
         return np.array(lines), s.get_stale_velocity()
 
 
-def _compare(n, nu, dt, steps, u, every=10):
-    (a, ua), (b, ub) = (_run(n, nu, dt, steps, u, p, every) for p in ("fp64", "mixed"))
-    de = np.max(np.abs(b[:, 0] - a[:, 0]) / a[:, 0])
-    dz = np.max(np.abs(b[:, 1] - a[:, 1]) / a[:, 1])
+def _dev(a, b):
+    (la, ua), (lb, ub) = a, b
+    de = np.max(np.abs(lb[:, 0] - la[:, 0]) / la[:, 0])
+    dz = np.max(np.abs(lb[:, 1] - la[:, 1]) / la[:, 1])
     du = float(np.max(np.abs(ub - ua)) / np.max(np.abs(ua)))
-    print(f"n={n} steps={steps}: max rel dev E {de:.2e} Z {dz:.2e}, max-norm dev u {du:.2e}", flush=True)
-    assert len(a) == steps // every + 1
-    assert de <= 1e-5 and dz <= 1e-5 and du <= 1e-3, (de, dz, du)
+    return de, dz, du
+
+
+def _compare(n, nu, dt, steps, u, every=10):
+    ref, mx, m64 = (_run(n, nu, dt, steps, u, p, every) for p in ("fp64", "mixed", "mixed_acc64"))
+    assert len(ref[0]) == steps // every + 1
+    devs = {}
+    for name, run in (("mixed", mx), ("mixed_acc64", m64)):
+        de, dz, du = devs[name] = _dev(ref, run)
+        print(f"n={n} steps={steps} {name}: max rel dev E {de:.2e} Z {dz:.2e}, max-norm dev u {du:.2e}",
+              flush=True)
+        assert de <= 1e-5 and dz <= 1e-5 and du <= 1e-3, (name, de, dz, du)
+    de, dz, du = _dev(m64, mx)
+    print(f"n={n} steps={steps} float vs double running sum: E {de:.2e} Z {dz:.2e} u {du:.2e}", flush=True)
+    assert de <= 1e-6 and dz <= 1e-6 and du <= 1e-4, (de, dz, du)
+    assert de <= devs["mixed"][0] and dz <= devs["mixed"][1] and du <= devs["mixed"][2], (de, dz, du, devs)
 
 
 @pytest.mark.parametrize("ic", ["tgv", "iso"])

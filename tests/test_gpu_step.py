@@ -149,7 +149,35 @@ def test_mixed_isotropic_field_max_norm(tmp_path):
         assert we < 1e-5 and wz < 1e-5 and wu < 1e-5
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_mixed_running_sum_in_double():
+    """DNS_MIXED_ACC64 differs from DNS_MIXED only in the running sum of the
+    stage increments (double instead of float): the two runs differ (the
+    double sum is in use) by no more than the float rounding of that sum,
+    and the double-sum run is no further from the FP64 run than the float
+    one by more than that rounding."""
+    n, nu, dt = 64, 0.01, 0.004
+    u = random_field(n, 7)
+    out = {}
+    for p in ("fp64", "mixed", "mixed_acc64"):
+        with dns_amd.Solver(n, nu, dt, p) as s:
+            s.load_velocity(u)
+            lines = [s.diagnostics()]
+            for _ in range(5):
+                s.step(10)
+                lines.append(s.diagnostics())
+            out[p] = (np.array(lines), s.get_spectral())
+    (e64, _), (ef, sf), (ed, sd) = out["fp64"], out["mixed"], out["mixed_acc64"]
+    assert not np.array_equal(sf, sd)
+    rel = np.max(np.abs(sf - sd)) / np.max(np.abs(sd))
+    print(f"float vs double running sum after 50 steps: spectral max-norm {rel:.2e}", flush=True)
+    assert rel <= 1e-6  # measured 9.1e-9
+    for name, e in (("mixed", ef), ("mixed_acc64", ed)):
+        de = np.max(np.abs(e[:, 0] - e64[:, 0]) / e64[:, 0])
+        print(f"{name}: max rel energy dev vs fp64 {de:.2e}", flush=True)
+        assert de <= 1e-5
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "mixed_acc64"])
 def test_state_transfer_continues_bitwise(precision):
     """A state taken out with get_spectral / get_stale_velocity and put into
     a second solver with set_spectral / set_stale_velocity continues bit for
@@ -171,7 +199,7 @@ def test_state_transfer_continues_bitwise(precision):
         assert np.array_equal(b.get_stale_velocity(), a.get_stale_velocity())
 
 
-@pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("mixed", 1e-6)])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("mixed", 1e-6), ("mixed_acc64", 1e-6)])
 def test_beltrami_flow_decays_by_rk4_polynomial(precision, tol):
     """curl u = k u, so u x omega = 0: the state decays by R(nu dt k^2)."""
     n, nu, dt, k, m = 32, 0.02, 0.05, 3, 20
@@ -184,7 +212,7 @@ def test_beltrami_flow_decays_by_rk4_polynomial(precision, tol):
         assert np.max(np.abs(s.get_spectral() - want)) / np.max(np.abs(u0)) < tol
 
 
-@pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("mixed", 1e-6)])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("mixed", 1e-6), ("mixed_acc64", 1e-6)])
 def test_divergence_decays_by_rk4_polynomial(precision, tol):
     """The projection removes the divergent part of N, so k.U of any field
     evolves by R(nu dt |k|^2) per step."""
