@@ -38,7 +38,7 @@ static __global__ void k_enstrophy_grad(const double2 *__restrict__ U, double2 *
   state_row(row, n, i, j);
   const double kx = i < n / 2 ? i : i - n, ky = j < n / 2 ? j : j - n;
   for (int k = threadIdx.x; k < g.nfp; k += blockDim.x) {
-    const long l = row * g.nfp + k;
+    const long l = srow(row, k, g);
     const double kk = kx * kx + ky * ky + (double)k * k;
 #pragma unroll
     for (int f = 0; f < 3; f++) {
@@ -48,12 +48,13 @@ static __global__ void k_enstrophy_grad(const double2 *__restrict__ U, double2 *
   }
 }
 
-// out = W^-1 in (spectral, 3 fields) -- used for the final R2C adjoint.
+// out = W^-1 in (spectral state layout, 3 fields) -- used for the final
+// R2C adjoint.
 static __global__ void k_hweight_inv(const double2 *__restrict__ in, double2 *__restrict__ out, int n, Geo g) {
   const int nf = n / 2 + 1;
   const long row = blockIdx.x;
   for (int k = threadIdx.x; k < nf; k += blockDim.x) {
-    const long l = row * g.nfp + k;
+    const long l = srow(row, k, g);
     const double s = 1.0 / hweight(k, n);
 #pragma unroll
     for (int f = 0; f < 3; f++) {

@@ -38,7 +38,7 @@ __device__ __forceinline__ void adj_pre_x(double2 (&d)[3][E], double2 *__restric
                                           const double2 *__restrict__ tw) {
 #pragma clang fp contract(off)
   constexpr int TPP = N / E;
-  const long soff = (long)j * N * g.nfp + k, sstr = g.nfp;
+  const long jrow = (long)j * N;  // state rows (srow)
   const double ky = wave<N>(j), kz = k;
   const bool zy = fabs(ky) < q.kmax && fabs(kz) < q.kmax;
   const double sN0 = q.dt / hweight(k, N);
@@ -52,7 +52,7 @@ __device__ __forceinline__ void adj_pre_x(double2 (&d)[3][E], double2 *__restric
     if (ok) {
 #pragma unroll
       for (int f = 0; f < 3; f++)
-        st<double>(lamV + f * g.fs + soff + i * sstr, make_double2(sU * d[f][c].x, sU * d[f][c].y));
+        st<double>(lamV + f * g.fs + srow(jrow + i, k, g), make_double2(sU * d[f][c].x, sU * d[f][c].y));
     }
     double2 pr[3];
     if (kk > 0) {
@@ -96,13 +96,13 @@ __global__ __launch_bounds__(T *(N / E)) void k_adj_xa(const double2 *__restrict
   const int p = threadIdx.x % T, t = threadIdx.x / T;
   const int j = blockIdx.y, k = blockIdx.x * T + p;
   const bool ok = k < N / 2 + 1;
-  const long soff = (long)j * N * g.nfp + k, sstr = g.nfp;
+  const long jrow = (long)j * N;  // state rows (srow)
   double2 d[3][E];
 #pragma unroll
   for (int f = 0; f < 3; f++)
 #pragma unroll
     for (int c = 0; c < E; c++) {
-      const double2 x = ok ? lamU[f * g.fs + soff + (t + TPP * c) * sstr] : make_double2(0, 0);
+      const double2 x = ok ? lamU[f * g.fs + srow(jrow + t + TPP * c, k, g)] : make_double2(0, 0);
       d[f][c] = make_double2(q.a * x.x, q.a * x.y);
     }
   adj_pre_x<N, T, E>(d, lamV, W, g, q, t, p, j, k, ok, lds, tw);
@@ -125,7 +125,7 @@ __global__ __launch_bounds__(T *(N / E)) void k_adj_xe(double2 *__restrict__ W, 
   const int j = blockIdx.y, k = blockIdx.x * T + p;
   const bool ok = k < N / 2 + 1;
   const long off = (long)j * g.nfp + k, istr = g.wps;          // partials
-  const long soff = (long)j * N * g.nfp + k, sstr = g.nfp;     // state layout
+  const long jrow = (long)j * N;                               // state rows (srow)
   const TileSlot<N, T> slot{p};
   double2 C[3][E];
 #pragma unroll
@@ -160,7 +160,7 @@ __global__ __launch_bounds__(T *(N / E)) void k_adj_xe(double2 *__restrict__ W, 
   const double wk = hweight(k, N);
 #pragma unroll
   for (int c = 0; c < E; c++) {
-    const long l = soff + (t + TPP * c) * sstr;
+    const long l = srow(jrow + t + TPP * c, k, g);
     double2 r[3], base[3], lu[3];
     if (ok) {
 #pragma unroll

@@ -103,6 +103,7 @@ namespace dnsk {
 
 struct Impl {
   virtual ~Impl() {}
+  virtual int state_pairs() const = 0;  // Geo::spair of this grid size
   virtual size_t sr_size() const = 0;  // bytes per complex element of uhr
   virtual size_t acc_bytes(long fs) const = 0;
   virtual size_t r_size() const = 0;   // bytes per real element of W/us
@@ -253,6 +254,9 @@ template <int N, typename R, typename SR, bool S64 = false> struct Engine final 
   // gains 10% but the step measured 1% slower, so 1)
   static constexpr int MINW_Z = 1;
 
+  // interleaved state row pairs (srow) where a pencil takes more than 64
+  // threads: the x pass's tiles are 4 double columns there
+  int state_pairs() const override { return TPP > 64; }
   size_t sr_size() const override { return sizeof(cx<SR>); }
   // bytes per member of the accumulator region: 3 double fields, or in
   // mixed precision the float copy of Un (3 float fields) and the running
@@ -340,7 +344,7 @@ template <int N, typename R, typename SR, bool S64 = false> struct Engine final 
     c->w_ready = 0;
     hipStream_t s = c->stream;
     double2 *ws = (double2 *)wslice(c, m);
-    k_swap_rows<double2><<<N * N, 256, 0, s>>>((const double2 *)c->uhn + m * c->g.mS, ws, N, c->g, 3);
+    k_state_rows<double2, false><<<N * N, 256, 0, s>>>((const double2 *)c->uhn + m * c->g.mS, ws, N, c->g, 3);
     DNS_LAUNCHED();
     DNS_HIP(hipMemcpy2DAsync(out, sizeof(double2) * NF, ws, sizeof(double2) * c->g.nfp, sizeof(double2) * NF,
                              (size_t)3 * N * N, hipMemcpyDeviceToHost, s));
@@ -356,7 +360,7 @@ template <int N, typename R, typename SR, bool S64 = false> struct Engine final 
     DNS_HIP(hipMemset2DAsync(ws, pitch, 0, pitch, (size_t)3 * N * N, s));
     DNS_HIP(hipMemcpy2DAsync(ws, pitch, in, sizeof(double2) * NF, sizeof(double2) * NF, (size_t)3 * N * N,
                              hipMemcpyHostToDevice, s));
-    k_swap_rows<double2><<<N * N, 256, 0, s>>>(ws, (double2 *)c->uhn + m * c->g.mS, N, c->g, 3);
+    k_state_rows<double2, true><<<N * N, 256, 0, s>>>(ws, (double2 *)c->uhn + m * c->g.mS, N, c->g, 3);
     DNS_LAUNCHED();
     int rc = state_changed(c, m);
     if (rc) return rc;
@@ -696,18 +700,18 @@ int Engine<N, R, SR, S64>::adj_step(dns_ctx *c, const double *uh, const double *
     // host spectra are x-major; tmpN takes them into the state layout
     DNS_HIP(hipMemset2DAsync(w->tmpN, pitch, 0, pitch, rows, s));
     DNS_HIP(hipMemcpy2DAsync(w->tmpN, pitch, uh, width, width, rows, hipMemcpyDefault, s));
-    k_swap_rows<double2><<<N * N, 256, 0, s>>>(w->tmpN, (double2 *)c->uhn, N, c->g, 3);
+    k_state_rows<double2, true><<<N * N, 256, 0, s>>>(w->tmpN, (double2 *)c->uhn, N, c->g, 3);
     DNS_LAUNCHED();
     DNS_HIP(hipMemcpyAsync(c->us, us, w->R3, hipMemcpyDefault, s));
     DNS_HIP(hipMemset2DAsync(w->tmpN, pitch, 0, pitch, rows, s));
     DNS_HIP(hipMemcpy2DAsync(w->tmpN, pitch, lam_uh, width, width, rows, hipMemcpyDefault, s));
-    k_swap_rows<double2><<<N * N, 256, 0, s>>>(w->tmpN, w->lamU, N, c->g, 3);
+    k_state_rows<double2, true><<<N * N, 256, 0, s>>>(w->tmpN, w->lamU, N, c->g, 3);
     DNS_LAUNCHED();
     DNS_HIP(hipMemcpyAsync(w->lamS, lam_us, w->R3, hipMemcpyDefault, s));
     c->w_ready = 0;
     if ((rc = Adjoint<N>::record(c, w->recs[0]))) return rc;
     if ((rc = Adjoint<N>::backward_stages(c, w, (const double *)w->recs[0]))) return rc;
-    k_swap_rows<double2><<<N * N, 256, 0, s>>>(w->lamU, w->tmpN, N, c->g, 3);
+    k_state_rows<double2, false><<<N * N, 256, 0, s>>>(w->lamU, w->tmpN, N, c->g, 3);
     DNS_LAUNCHED();
     DNS_HIP(hipMemcpy2DAsync(out_uh, width, w->tmpN, pitch, width, rows, hipMemcpyDefault, s));
     DNS_HIP(hipMemcpyAsync(out_us, w->lamS, w->R3, hipMemcpyDefault, s));

@@ -79,6 +79,14 @@ template <int COLS, typename T> __device__ __forceinline__ T ld_row(const T *p) 
   if constexpr (COLS * sizeof(T) >= 128) return ldnt(p);
   else return *p;
 }
+// Load of one element of a state-layout tile row: non-temporal when the
+// tile covers whole lines, one row per line or, with interleaved row pairs
+// (pairs set; see srow), two rows per line.
+template <int COLS, typename T> __device__ __forceinline__ T ld_state(const T *p, int pairs) {
+  if constexpr (COLS * sizeof(T) >= 128) return ldnt(p);
+  else if constexpr (2 * COLS * sizeof(T) >= 128) return pairs ? ldnt(p) : *p;
+  else return *p;
+}
 template <typename R> __device__ __forceinline__ void st_real(R *p, R v) {
 #if DNS_NT_STORES
   __builtin_nontemporal_store(v, p);
@@ -90,9 +98,14 @@ template <typename R> __device__ __forceinline__ void st_real(R *p, R v) {
 // ---------------------------------------------------------------------------
 // Radix schedules.  E = elements per thread (divisible by every radix), so
 // a pencil takes N/E threads; radix-3 grids keep E at 6 or 12 (registers).
-template <int N> struct Sched;
+// The radix order of each size is the fastest of the fewest-stage orderings
+// timed by dns_amd/tune_radix.py (one build per candidate); a build may
+// replace one size's entry with -DDNS_SCHED_N=n -DDNS_SCHED_E=e
+// -DDNS_SCHED_R=r0,r1,... (the tuner's candidate builds).
+template <int N> struct SchedTable;
+template <int N> struct Sched : SchedTable<N> {};
 #define DNS_SCHED(N_, E_, ...)                                \
-  template <> struct Sched<N_> {                              \
+  template <> struct SchedTable<N_> {                         \
     static constexpr int E = E_;                              \
     static constexpr int r[] = {__VA_ARGS__};                 \
     static constexpr int NS = sizeof(r) / sizeof(r[0]);       \
@@ -113,9 +126,9 @@ DNS_SCHED(108, 12, 4, 3, 3, 3)
 DNS_SCHED(128, 8, 8, 4, 4)
 DNS_SCHED(144, 12, 4, 4, 3, 3)
 DNS_SCHED(162, 6, 2, 3, 3, 3, 3)
-DNS_SCHED(192, 12, 4, 4, 4, 3)
+DNS_SCHED(192, 12, 4, 4, 3, 4)
 DNS_SCHED(216, 12, 4, 2, 3, 3, 3)
-DNS_SCHED(256, 8, 8, 8, 4)
+DNS_SCHED(256, 8, 8, 4, 8)
 DNS_SCHED(288, 12, 4, 4, 2, 3, 3)
 DNS_SCHED(324, 12, 4, 3, 3, 3, 3)
 DNS_SCHED(384, 12, 4, 4, 4, 2, 3)
@@ -129,6 +142,13 @@ DNS_SCHED(864, 12, 4, 4, 2, 3, 3, 3)
 DNS_SCHED(972, 12, 4, 3, 3, 3, 3, 3)
 DNS_SCHED(1024, 8, 8, 8, 4, 4)
 #undef DNS_SCHED
+#ifdef DNS_SCHED_N
+template <> struct Sched<DNS_SCHED_N> {
+  static constexpr int E = DNS_SCHED_E;
+  static constexpr int r[] = {DNS_SCHED_R};
+  static constexpr int NS = sizeof(r) / sizeof(r[0]);
+};
+#endif
 
 template <int N> constexpr int stage_Ns(int s) {
   int ns = 1;

@@ -6,11 +6,13 @@
 //                     main.c:236,358; one spare row per x-plane),
 //                     k < nf = n/2+1 valid, nfp = nf rounded up to 16
 //                     (rows start on 128-byte boundaries);
-//   spectral state (uhn, uhr, acc, phat): complex [n][n][nfp] with y
-//                     slowest, element (i,j,k) at (j*n+i)*nfp+k: only the
-//                     x pass touches the state during a step, and in this
-//                     layout its x-pencils have the short row stride of a
-//                     y pass (float x pass 2.72 -> 2.39 ms at n=512);
+//   spectral state (uhn, uhr, acc, phat): complex rows of nfp, row
+//                     j*n+i (y slowest), element k at srow(row, k): only
+//                     the x pass touches the state during a step, and in
+//                     this layout its x-pencils have the short row stride of
+//                     a y pass (float x pass 2.72 -> 2.39 ms at n=512); at
+//                     n=1024 rows i, i+1 are interleaved so that a tile of
+//                     4 double columns covers whole 128-byte lines;
 //   physical field  : real [n][n][n], (i*n+j)*n+k.
 // A "pencil" is the set of elements along one axis with the other two
 // indices fixed.  x and y pencils are strided: a block owns T pencils with
@@ -49,6 +51,7 @@ struct Geo {
   // with one spare row 5.1 TB/s); wfs = n*wps per field.  Scratch uses of
   // the W buffer (transforms of the state, snapshots) keep the fs layout.
   long wfs, wps;
+  int spair;  // spectral state rows interleaved in pairs (srow)
 };
 
 // z pencils per block of the contiguous-axis passes: as many as fill one
@@ -107,16 +110,38 @@ __device__ __forceinline__ void state_row(long r, int n, int &i, int &j) {
   j = (int)(r / n);
 }
 
-// Swaps the two row indices of `nfields` padded spectral fields:
-// dst[(b*n+a)*nfp+k] = src[(a*n+b)*nfp+k] (x-major <-> state layout; the
-// map is its own inverse).  Grid (n*n) blocks.
-template <typename T>
-__global__ __launch_bounds__(256) void k_swap_rows(const T *__restrict__ src, T *__restrict__ dst, int n, Geo g,
-                                                   int nfields) {
-  const long r = blockIdx.x, a = r / n, b = r % n;
+// Offset of element k of row r in a spectral state field (uhn, uhr, acc,
+// phat and the adjoint's state-shaped arrays): r*nfp + k, or, where g.spair
+// is set, with rows 2q and 2q+1 (x rows i, i+1 of one y plane j, n being
+// even) interleaved in groups of four complex elements: row r's group m at
+// (r & ~1)*nfp + 8m + 4(r & 1).  A 128-byte line of double elements then
+// holds columns 4m..4m+3 of two x rows (of float elements 4m..4m+7), so an
+// x-pass tile of 4 double columns reads and writes whole lines; with one
+// row per line each line is split between two neighbouring tiles' blocks.
+// The engine sets spair where a pencil takes more than 64 threads (n=1024:
+// 128, so a 512-thread block holds 4 double columns): there the x pass
+// measured 39.5 -> 30.5 ms (FP64) and 24.1 -> 21.0 ms (mixed).  With 8-
+// column tiles (n=512) the interleaved rows measured slower (x pass 3.40 ->
+// 3.80 ms), so smaller grids keep one row per line.  nfp is a multiple of 4.
+__device__ __forceinline__ long srow(long r, int k, const Geo &g) {
+  return g.spair ? (r & ~1L) * g.nfp + ((k >> 2) << 3) + ((r & 1) << 2) + (k & 3) : r * g.nfp + k;
+}
+
+// Converts `nfields` padded spectral fields between the x-major layout of
+// the host API (row i*n + j, element k at (i*n + j)*nfp + k) and the state
+// layout (row j*n + i, srow): TO_STATE, dst (state) <- src (x-major);
+// otherwise dst (x-major) <- src (state).  Grid (n*n) blocks, block r =
+// state row j*n + i.
+template <typename T, bool TO_STATE>
+__global__ __launch_bounds__(256) void k_state_rows(const T *__restrict__ src, T *__restrict__ dst, int n, Geo g,
+                                                    int nfields) {
+  const long r = blockIdx.x, j = r / n, i = r % n;
+  const long x = (i * n + j) * g.nfp;
   for (int f = 0; f < nfields; f++)
-    for (int k = threadIdx.x; k < g.nfp; k += blockDim.x)
-      dst[f * g.fs + (b * n + a) * g.nfp + k] = src[f * g.fs + r * g.nfp + k];
+    for (int k = threadIdx.x; k < g.nfp; k += blockDim.x) {
+      if (TO_STATE) dst[f * g.fs + srow(r, k, g)] = src[f * g.fs + x + k];
+      else dst[f * g.fs + x + k] = src[f * g.fs + srow(r, k, g)];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -228,19 +253,19 @@ __global__ __launch_bounds__(T *(N / Sched<N>::E)) void k_xplain(const cx<RI> *i
   const int p = threadIdx.x % T, t = threadIdx.x / T;
   const int k = blockIdx.x * T + p;
   const bool ok = k < N / 2 + 1;
-  const long off = (long)blockIdx.y * g.nfp + k, istr = (long)N * g.nfp;
-  const long soff = (long)blockIdx.y * N * g.nfp + k, sstr = g.nfp;
-  const long ioff = SIN ? soff : off, iistr = SIN ? sstr : istr;
-  const long ooff = SOUT ? soff : off, oistr = SOUT ? sstr : istr;
+  // element at x index i: x-major (i*n + j)*nfp + k, state srow(j*n + i, k)
+  const long j = blockIdx.y;
+  auto at = [&](bool state, int i) {
+    return state ? srow(j * N + i, k, g) : ((long)i * N + j) * g.nfp + k;
+  };
   for (int f = 0; f < nfields; f++) {
     cx<R> v[E];
 #pragma unroll
-    for (int c = 0; c < E; c++)
-      v[c] = ok ? cvt<R, RI>(in[f * g.fs + ioff + (t + TPP * c) * iistr]) : zero<R>();
+    for (int c = 0; c < E; c++) v[c] = ok ? cvt<R, RI>(in[f * g.fs + at(SIN, t + TPP * c)]) : zero<R>();
     fft_pencil<N, R, INV>(v, t, lds, TileSlot<N, T>{p}, tw);
     if (ok) {
 #pragma unroll
-      for (int c = 0; c < E; c++) st<R>(out + f * g.fs + ooff + (t + TPP * c) * oistr, v[c]);
+      for (int c = 0; c < E; c++) st<R>(out + f * g.fs + at(SOUT, t + TPP * c), v[c]);
     }
   }
 }
@@ -321,14 +346,13 @@ __global__ __launch_bounds__(T *(N / E), MINW) void k_xinv_curl(const cx<SI> *__
   const bool ok = k < N / 2 + 1;
   uh += blockIdx.z * g.mS;
   W += blockIdx.z * g.mW;
-  const long soff = (long)j * N * g.nfp + k;  // state layout
   cx<R> U[E], V[E], Wv[E];
 #pragma unroll
   for (int c = 0; c < E; c++) {
-    const long l = soff + (t + TPP * c) * g.nfp;
-    U[c] = ok ? cvt<R, SI>(ld_row<T>(uh + l)) : zero<R>();
-    V[c] = ok ? cvt<R, SI>(ld_row<T>(uh + g.fs + l)) : zero<R>();
-    Wv[c] = ok ? cvt<R, SI>(ld_row<T>(uh + 2 * g.fs + l)) : zero<R>();
+    const long l = srow((long)j * N + t + TPP * c, k, g);
+    U[c] = ok ? cvt<R, SI>(ld_state<T>(uh + l, g.spair)) : zero<R>();
+    V[c] = ok ? cvt<R, SI>(ld_state<T>(uh + g.fs + l, g.spair)) : zero<R>();
+    Wv[c] = ok ? cvt<R, SI>(ld_state<T>(uh + 2 * g.fs + l, g.spair)) : zero<R>();
   }
   curl_inverse_x<N, R, T, WITH_U, E>(U, V, Wv, W, g, t, p, j, k, ok, lds, tw);
 }
@@ -678,8 +702,8 @@ __device__ __forceinline__ void forward_x_rk(const cx<R> *__restrict__ W, cx<SR>
 #pragma clang fp contract(off)
   constexpr int TPP = N / E;
   const long off = (long)j * g.nfp + k, istr = g.wps;  // step partials
-  // state element (i, j, k) at soff + i * sstr (state layout)
-  const long soff = (long)j * N * g.nfp + k, sstr = g.nfp;
+  // state element (i, j, k) at srow(jrow + i, k, g) (state layout)
+  const long jrow = (long)j * N;
   const TileSlot<N, T> slot{p};
   constexpr bool MIXED_STATE = sizeof(SR) == 4;
   // two positions per batch (six or nine loads; one position for
@@ -745,18 +769,18 @@ __device__ __forceinline__ void forward_x_rk(const cx<R> *__restrict__ W, cx<SR>
     for (int i = 0; i < BC; i++) {
 #pragma unroll
       for (int f = 0; f < 3; f++) {
-        const long lf = f * g.fs + soff + (t + TPP * (c0 + i)) * sstr;
-        if constexpr (STAGE == 0 && MIXED_STATE) B.ur[f][i] = ld_row<T>(inc + lf);
-        else if constexpr (STAGE == 0) B.ur[f][i] = ld_row<T>(uhn + lf);
-        else B.ur[f][i] = ld_row<T>(uhr + lf);
+        const long lf = f * g.fs + srow(jrow + t + TPP * (c0 + i), k, g);
+        if constexpr (STAGE == 0 && MIXED_STATE) B.ur[f][i] = ld_state<T>(inc + lf, g.spair);
+        else if constexpr (STAGE == 0) B.ur[f][i] = ld_state<T>(uhn + lf, g.spair);
+        else B.ur[f][i] = ld_state<T>(uhr + lf, g.spair);
         if constexpr (MIXED_STATE) {
-          if constexpr (STAGE == 1 || STAGE == 2) B.unf[f][i] = ld_row<T>(inc + lf);
-          if constexpr (STAGE == 3) B.un[f][i] = ld_row<T>(uhn + lf);
-          if constexpr (STAGE > 0 && S64) B.sf[f][i] = ld_row<T>(s64 + lf);
-          else if constexpr (STAGE > 0) B.sf[f][i] = ld_row<T>(inc + 3 * g.fs + lf);
+          if constexpr (STAGE == 1 || STAGE == 2) B.unf[f][i] = ld_state<T>(inc + lf, g.spair);
+          if constexpr (STAGE == 3) B.un[f][i] = ld_state<T>(uhn + lf, g.spair);
+          if constexpr (STAGE > 0 && S64) B.sf[f][i] = ld_state<T>(s64 + lf, g.spair);
+          else if constexpr (STAGE > 0) B.sf[f][i] = ld_state<T>(inc + 3 * g.fs + lf, g.spair);
         } else {
-          if constexpr (STAGE == 1 || STAGE == 2) B.un[f][i] = ld_row<T>(uhn + lf);
-          if constexpr (STAGE > 0) B.ac[f][i] = ld_row<T>(acc + lf);
+          if constexpr (STAGE == 1 || STAGE == 2) B.un[f][i] = ld_state<T>(uhn + lf, g.spair);
+          if constexpr (STAGE > 0) B.ac[f][i] = ld_state<T>(acc + lf, g.spair);
         }
       }
     }
@@ -768,7 +792,7 @@ __device__ __forceinline__ void forward_x_rk(const cx<R> *__restrict__ W, cx<SR>
       const int c = c0 + i;
       U[c] = V[c] = Wv[c] = zero<R>();
       if (!ok) continue;
-      const long l = soff + (t + TPP * c) * sstr;
+      const long l = srow(jrow + t + TPP * c, k, g);
       const double kx = (double)wave<N>(t + TPP * c);
       const double kk = kx * kx + ky * ky + kz * kz;
       const double fac = (myz && fabs(kx) < q.kmax) ? dt : 0.0;
@@ -948,9 +972,9 @@ __global__ __launch_bounds__(BS) void k_diag(const double2 *__restrict__ uh, int
     int i, j;
     state_row(row, n, i, j);
     const double kx = i < n / 2 ? i : i - n, ky = j < n / 2 ? j : j - n;
-    const double2 *r = uh + (XMAJOR ? (long)i * n + j : row) * g.nfp;
     for (int k = threadIdx.x; k < nf; k += BS) {
-      const double2 a = r[k], b = r[g.fs + k], c = r[2 * g.fs + k];
+      const double2 *r = uh + (XMAJOR ? ((long)i * n + j) * g.nfp + k : srow(row, k, g));
+      const double2 a = r[0], b = r[g.fs], c = r[2 * g.fs];
       const double e = (a.x * a.x + a.y * a.y) + (b.x * b.x + b.y * b.y) + (c.x * c.x + c.y * c.y);
       const double kk = kx * kx + ky * ky + (double)k * (double)k;
       two_sum_acc(es, ec, e);
