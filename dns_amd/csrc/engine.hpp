@@ -274,7 +274,7 @@ template <int N, typename R, typename SR, bool S64 = false> struct Engine final 
   // grid of the x-pass kernels (k_xinv_curl, k_xfwd_rk_xinv): one-dimensional
   // where x_tile pairs the tiles of a line
   template <int T> dim3 grid_xp(int m = 1) const {
-    if constexpr (x_tiles_paired<T, R>()) return dim3((NF + T - 1) / T * N, 1, m);
+    if constexpr (x_tiles_paired<T, R>()) return dim3(paired_blocks((NF + T - 1) / T, N), 1, m);
     else return grid_x(T, m);
   }
 
@@ -467,10 +467,15 @@ template <int N, typename R, typename SR, bool S64 = false> struct Engine final 
     const R invn3 = (R)c->invn3;
     static const double a_rk[4] = {1 / 6.0, 1 / 3.0, 1 / 3.0, 1 / 6.0};  // main.c:15
     static const double b_rk[4] = {0.5, 0.5, 1.0, 0.0};                 // main.c:16
-    const dim3 gy = grid_x(TY, M), by(TY * TPP_Y);
+    // y passes: paired tiles where a tile row is narrower than a line
+    constexpr bool PY = x_tiles_paired<TY, R>(), PYF = x_tiles_paired<TYF, R>();
+    const dim3 gy = PY ? dim3(paired_blocks((NF + TY - 1) / TY, N), 1, M) : grid_x(TY, M), by(TY * TPP_Y);
     const dim3 gx = grid_xp<TX>(M), bx(TX * TPP_X);
     const dim3 gz(N * N / PP, 1, M), bz(PP * TPP);
-    const dim3 gy_trunc((g.nk + TYF - 1) / TYF, N, M), byf(TYF * TPP_Y);  // dealiased kz columns only
+    // the forward y pass covers the dealiased kz columns only
+    const dim3 gy_trunc = PYF ? dim3(paired_blocks((g.nk + TYF - 1) / TYF, N), 1, M)
+                              : dim3((g.nk + TYF - 1) / TYF, N, M),
+               byf(TYF * TPP_Y);
     for (long st = 0; st < nsteps; st++) {
       double2 *ph = (c->capture && st == nsteps - 1) ? (double2 *)c->phat : nullptr;
       for (int rk = 0; rk < 4; rk++) {
@@ -480,16 +485,16 @@ template <int N, typename R, typename SR, bool S64 = false> struct Engine final 
         // the adjoint's recording of this stage's u and omega (FP64, M = 1)
         double *rp = (sizeof(R) == 8 && c->rec_phys) ? c->rec_phys + (size_t)rk * 6 * g.rs : nullptr;
         if (rk == 0) {
-          DNS_TIMED(c, sl + 1, (k_ypass<N, R, TY, true, 1, PIPE_Y, false, E_Y, false, true><<<gy, by, 0, s>>>(W + 3 * g.wfs, 3, g, tw, g.mW)));
+          DNS_TIMED(c, sl + 1, (k_ypass<N, R, TY, true, 1, PIPE_Y, false, E_Y, false, true, PY><<<gy, by, 0, s>>>(W + 3 * g.wfs, 3, g, tw, g.mW)));
           if (rp)
             DNS_TIMED(c, sl + 2, (k_zfused<N, R, PP, 0, R, MINW_Z, REC_OK><<<gz, bz, 0, s>>>(W, (R *)c->us, g, invn3, tw, rp)));
           else
             DNS_TIMED(c, sl + 2, (k_zfused<N, R, PP, 0, R, MINW_Z><<<gz, bz, 0, s>>>(W, (R *)c->us, g, invn3, tw)));
         } else {
           if constexpr (CX_IN_Y)
-            DNS_TIMED(c, sl + 1, (k_ypass<N, R, TY, true, 1, PIPE_YC, false, E_Y, true, true><<<gy, by, 0, s>>>(W, 6, g, tw, g.mW)));
+            DNS_TIMED(c, sl + 1, (k_ypass<N, R, TY, true, 1, PIPE_YC, false, E_Y, true, true, PY><<<gy, by, 0, s>>>(W, 6, g, tw, g.mW)));
           else
-            DNS_TIMED(c, sl + 1, (k_ypass<N, R, TY, true, 1, PIPE_Y, false, E_Y, false, true><<<gy, by, 0, s>>>(W, 6, g, tw, g.mW)));
+            DNS_TIMED(c, sl + 1, (k_ypass<N, R, TY, true, 1, PIPE_Y, false, E_Y, false, true, PY><<<gy, by, 0, s>>>(W, 6, g, tw, g.mW)));
           if (rp && rk < 3)
             DNS_TIMED(c, sl + 2, (k_zfused<N, R, PP, 1, R, MINW_Z, REC_OK><<<gz, bz, 0, s>>>(W, (R *)c->us, g, invn3, tw, rp)));
           else if (rp)
@@ -500,7 +505,7 @@ template <int N, typename R, typename SR, bool S64 = false> struct Engine final 
             DNS_TIMED(c, sl + 2, (k_zfused<N, R, PP, 2, R, MINW_Z><<<gz, bz, 0, s>>>(W, (R *)c->us, g, invn3, tw)));
         }
         DNS_TIMED(c, sl + 3,
-                  (k_ypass<N, R, TYF, false, 1, PIPE_YF, true, E_Y, false, true><<<gy_trunc, byf, 0, s>>>(W, 3, g, tw, g.mW)));
+                  (k_ypass<N, R, TYF, false, 1, PIPE_YF, true, E_Y, false, true, PYF><<<gy_trunc, byf, 0, s>>>(W, 3, g, tw, g.mW)));
         // E fused with the next stage's A (the next step's stage 1 after rk 3)
         switch (rk) {
           case 0:

@@ -64,24 +64,31 @@ template <int N> constexpr int z_pencils_per_block() {
   return p;
 }
 
-// Column tile (bx) and row (by) of an x-pass block.  Tiles whose rows are
-// narrower than a 128-byte line (T complex elements) split each line between
-// two neighbouring tiles; such grids are launched one-dimensional and block
-// p takes the tile ordered so that the two tiles of a line run 8 blocks
-// apart, which the hardware dispatches to the same XCD (consecutive
-// workgroups go to the eight XCDs in turn): the second tile then finds the
-// line in that XCD's L2 instead of fetching it again.  ntx = column tiles
-// per row, nty = rows.
+// Column tile (bx) and row (by) of a strided-pass block.  Tiles whose rows
+// are narrower than a 128-byte line (T complex elements) split each line
+// between tiles 2q and 2q+1 of a row; such grids are launched one-
+// dimensional (paired_blocks) and block p takes member p%16 >= 8 of pair
+// slot 8*(p/16) + p%8 (slot = row * pairs per row + q), so the two tiles of a
+// line run 8 blocks apart, which the hardware dispatches to the same XCD
+// (consecutive workgroups go to the eight XCDs in turn): the second tile
+// then finds the line in that XCD's L2 instead of fetching it again.  Pairs
+// are formed within each row, also when a row has an odd number of tiles.
+// ntx = column tiles per row, nty = rows.  Returns false for the blocks past
+// the last tile (the launch is rounded up to whole groups of 16): they exit
+// before their first barrier.
 template <int T, typename R> constexpr bool x_tiles_paired() { return T * sizeof(cx<R>) < 128; }
-template <int T, typename R> __device__ __forceinline__ void x_tile(int ntx, int nty, int &bx, int &by) {
+__host__ __device__ constexpr long paired_blocks(long ntx, long nty) { return ((ntx + 1) / 2 * nty + 7) / 8 * 16; }
+template <int T, typename R> __device__ __forceinline__ bool x_tile(int ntx, int nty, int &bx, int &by) {
   if constexpr (x_tiles_paired<T, R>()) {
-    const long P = (long)ntx * nty, p = blockIdx.x;
-    const long l = p < P / 16 * 16 ? 2 * (8 * (p / 16) + p % 8) + ((p % 16) >= 8 ? 1 : 0) : p;
-    bx = (int)(l % ntx);
-    by = (int)(l / ntx);
+    const int p = blockIdx.x, np = (ntx + 1) / 2;
+    const int slot = 8 * (p / 16) + p % 8;
+    by = slot / np;
+    bx = 2 * (slot % np) + ((p % 16) >= 8 ? 1 : 0);
+    return by < nty && bx < ntx;
   } else {
     bx = blockIdx.x;
     by = blockIdx.y;
+    return true;
   }
 }
 
@@ -159,8 +166,9 @@ __global__ __launch_bounds__(256) void k_state_rows(const T *__restrict__ src, T
 // from fields 1 and 2 as loaded, then transformed and stored as field 3.
 // In (x, ky, kz) space ky and kz are per-element constants, so this equals
 // the inverse x transform of the spectral curlX.
+// PAIR: one-dimensional launch of paired tiles (x_tile; paired_blocks).
 template <int N, typename R, int T, bool INV, int MINW = 1, int PIPE = 0, bool TRUNC = false,
-          int E = Sched<N>::E, bool CURLX = false, bool WP = false>
+          int E = Sched<N>::E, bool CURLX = false, bool WP = false, bool PAIR = false>
 __global__ __launch_bounds__(T *(N / E), MINW) void k_ypass(cx<R> *__restrict__ data, int nfields, Geo g,
                                                            const cx<R> *__restrict__ tw, long mstride = 0) {
   constexpr int TPP = N / E;
@@ -173,11 +181,15 @@ __global__ __launch_bounds__(T *(N / E), MINW) void k_ypass(cx<R> *__restrict__ 
   constexpr bool NTL = !TRUNC || sizeof(R) == 8;
   __shared__ cx<R> lds[T * padlen<N>()];
   __shared__ cx<R> twl[tw_size<N>()];
+  int bx = blockIdx.x, plane = blockIdx.y;
+  if constexpr (PAIR) {
+    if (!x_tile<T, R>(((TRUNC ? g.nk : N / 2 + 1) + T - 1) / T, N, bx, plane)) return;
+  }
   tw = stage_twiddles<N, R>(twl, tw);
   const int p = threadIdx.x % T, t = threadIdx.x / T;
-  const int k = blockIdx.x * T + p;
+  const int k = bx * T + p;
   const bool ok = k < (TRUNC ? g.nk : N / 2 + 1);
-  cx<R> *base = data + blockIdx.z * mstride + (long)blockIdx.y * pstr + k;
+  cx<R> *base = data + blockIdx.z * mstride + (long)plane * pstr + k;
   // loaded fields: all, or 0,1,2,4,5 with CURLX
   const int nload = CURLX ? 5 : nfields;
   auto fld = [](int i) { return CURLX && i >= 3 ? i + 1 : i; };
@@ -338,10 +350,10 @@ __global__ __launch_bounds__(T *(N / E), MINW) void k_xinv_curl(const cx<SI> *__
   constexpr int TPP = N / E;
   __shared__ cx<R> lds[T * padlen<N>()];
   __shared__ cx<R> twl[tw_size<N>()];
+  int bx, j;
+  if (!x_tile<T, R>((N / 2 + T) / T, N, bx, j)) return;
   tw = stage_twiddles<N, R>(twl, tw);
   const int p = threadIdx.x % T, t = threadIdx.x / T;
-  int bx, j;
-  x_tile<T, R>((N / 2 + T) / T, N, bx, j);
   const int k = bx * T + p;
   const bool ok = k < N / 2 + 1;
   uh += blockIdx.z * g.mS;
@@ -934,10 +946,10 @@ __global__ __launch_bounds__(T *(N / E), MINW) void k_xfwd_rk_xinv(cx<R> *__rest
                                                                        RKParams q, const cx<R> *__restrict__ tw) {
   __shared__ cx<R> lds[T * padlen<N>()];
   __shared__ cx<R> twl[tw_size<N>()];
+  int bx, j;
+  if (!x_tile<T, R>((N / 2 + T) / T, N, bx, j)) return;
   tw = stage_twiddles<N, R>(twl, tw);
   const int p = threadIdx.x % T, t = threadIdx.x / T;
-  int bx, j;
-  x_tile<T, R>((N / 2 + T) / T, N, bx, j);
   const int k = bx * T + p;
   const bool ok = k < N / 2 + 1;
   cx<R> U[E], V[E], Wv[E];
