@@ -250,8 +250,8 @@ template <int N, typename R, typename SR, bool S64 = false> struct Engine final 
   // recording kernels exist for FP64 only (the adjoint's precision)
   static constexpr bool REC_OK = sizeof(R) == 8;
   // minimum waves per SIMD the z pass's registers must allow (n=1024 FP64:
-  // 2 instead of 1 spills 92-124 bytes per lane in stages 2-4; the z pass
-  // gains 10% but the step measured 1% slower, so 1)
+  // 2 instead of 1 spills 92-124 bytes per lane in stages 2-4 and measured
+  // no faster, 20.4 against 20.0-20.2 ms per launch in same-box runs)
   static constexpr int MINW_Z = 1;
 
   // interleaved state row pairs (srow) where a pencil takes more than 64

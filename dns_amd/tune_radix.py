@@ -84,17 +84,17 @@ def build(sizes, jobs=8):
             f.write(f"#pragma once\n#define DNS_GRID_SIZES(X) X({n})\n")
         for fn in sorted(os.listdir(src)):
             if fn.endswith(".hip") and fn != "inst.hip":
-                work.append(("common", n, None, os.path.join(src, fn), os.path.join(common, fn[:-4] + ".o"), []))
+                work.append((os.path.join(src, fn), os.path.join(common, fn[:-4] + ".o"), []))
         for rad in cands:
             d = os.path.join(TUNE, name(n, rad))
             os.makedirs(d, exist_ok=True)
             flags = [f"-DDNS_N={n}", f"-DDNS_SCHED_N={n}", f"-DDNS_SCHED_E={e}",
                      "-DDNS_SCHED_R=" + ",".join(map(str, rad))]
-            work.append(("inst", n, rad, os.path.join(src, "inst.hip"), os.path.join(d, "inst.o"), flags))
+            work.append((os.path.join(src, "inst.hip"), os.path.join(d, "inst.o"), flags))
+
     def cc(w):
-        kind, n, rad, s, o, flags = w
+        s, o, flags = w
         B._run([B.HIPCC, *B.HIP_FLAGS, *flags, "-c", s, "-o", o])
-        return w
 
     with cf.ThreadPoolExecutor(jobs) as ex:
         list(ex.map(cc, work))
