@@ -12,6 +12,9 @@ schedules, pencil widths and z-pass blocks:
   1-D spectra (a closed form: every coefficient is a product of three 1-D
   DFT values), and the Beltrami flow's decay by the RK4 polynomial in FP64
   and mixed precision.
+* n=864 (spectral state rows interleaved in pairs): the state of a
+  separable field through dns_get_spectral / dns_set_spectral against its
+  closed form.
 * `dns-ic -l 3` (tgv.py -l 3: n = 8) through the command line, 1000 steps,
   against the oracle command line."""
 import math
@@ -205,6 +208,38 @@ def test_large_beltrami(n, precision, tol):
         e, z = s.diagnostics()
         f = rk4_amplification(nu * dt * k * k) ** (2 * m)
         assert abs(e - e0 * f) / (e0 * f) < tol and abs(z - z0 * f) / (z0 * f) < tol
+
+
+def test_interleaved_state_rows_round_trip():
+    """n=864 keeps its spectral state with x rows interleaved in pairs (a
+    pencil takes 72 threads; x tiles are 7 columns wide).  The state of a
+    loaded separable field, read back through dns_get_spectral, holds the
+    closed-form coefficients at their (i, j, k) positions; written back with
+    dns_set_spectral it reads back bitwise; and the device's enstrophy sum,
+    which weights every position by its |k|^2, equals the sum over the
+    closed form."""
+    n = 864
+    u, coef = _separable(n, seed=7)
+    print(f"interleaved state rows n={n}", flush=True)
+    with dns_amd.Solver(n, 0.01, 0.01, "fp64") as s:
+        s.load_velocity(np.stack([u, u, u]))
+        del u
+        e, z = s.diagnostics()
+        spec = s.get_spectral()
+        scale = max(abs(v) for v in coef.values())
+        for (i, j, k), v in coef.items():
+            assert np.all(np.abs(spec[:, i, j, k] - v) < 1e-13 * scale), (i, j, k)
+        s.set_spectral(spec)
+        back = s.get_spectral()
+        assert np.array_equal(back, spec)
+        del back, spec
+    ew = ze = 0.0
+    for (i, j, k), v in coef.items():   # plain half-spectrum sums over n^6 (main.c:257-266)
+        kx, ky = (i if i < n // 2 else i - n), (j if j < n // 2 else j - n)
+        a = 3 * abs(v) ** 2
+        ew += a
+        ze += (kx * kx + ky * ky + k * k) * a
+    assert abs(e - ew / n ** 6) / e < 1e-12 and abs(z - ze / n ** 6) / z < 1e-12, (e, ew / n ** 6, z, ze / n ** 6)
 
 
 def test_cli_level3_matches_oracle(tmp_path):
