@@ -91,7 +91,6 @@ static int ctx_create(dns_ctx **out, int n, int members, const double *nu, const
   c->g.mW = 6 * c->g.wfs;
   c->g.mS = 3 * c->g.fs;
   c->g.mR = 3 * c->g.rs;
-  c->g.spair = impl->state_pairs();
   if (hipSetDevice(device) != hipSuccess) {
     delete c->impl;
     delete c;

@@ -103,7 +103,6 @@ namespace dnsk {
 
 struct Impl {
   virtual ~Impl() {}
-  virtual int state_pairs() const = 0;  // Geo::spair of this grid size
   virtual size_t sr_size() const = 0;  // bytes per complex element of uhr
   virtual size_t acc_bytes(long fs) const = 0;
   virtual size_t r_size() const = 0;   // bytes per real element of W/us
@@ -254,9 +253,6 @@ template <int N, typename R, typename SR, bool S64 = false> struct Engine final 
   // no faster, 20.4 against 20.0-20.2 ms per launch in same-box runs)
   static constexpr int MINW_Z = 1;
 
-  // interleaved state row pairs (srow) where a pencil takes more than 64
-  // threads: the x pass's tiles are 4 double columns there
-  int state_pairs() const override { return TPP > 64; }
   size_t sr_size() const override { return sizeof(cx<SR>); }
   // bytes per member of the accumulator region: 3 double fields, or in
   // mixed precision the float copy of Un (3 float fields) and the running

@@ -79,12 +79,10 @@ template <int COLS, typename T> __device__ __forceinline__ T ld_row(const T *p) 
   if constexpr (COLS * sizeof(T) >= 128) return ldnt(p);
   else return *p;
 }
-// Load of one element of a state-layout tile row: non-temporal when the
-// tile covers whole lines, one row per line or, with interleaved row pairs
-// (pairs set; see srow), two rows per line.
-template <int COLS, typename T> __device__ __forceinline__ T ld_state(const T *p, int pairs) {
-  if constexpr (COLS * sizeof(T) >= 128) return ldnt(p);
-  else if constexpr (2 * COLS * sizeof(T) >= 128) return pairs ? ldnt(p) : *p;
+// Load of one element of a state-layout tile row (two x rows share each
+// line; see srow): non-temporal when the tile's two rows cover whole lines.
+template <int COLS, typename T> __device__ __forceinline__ T ld_state(const T *p) {
+  if constexpr (2 * COLS * sizeof(T) >= 128) return ldnt(p);
   else return *p;
 }
 template <typename R> __device__ __forceinline__ void st_real(R *p, R v) {
@@ -121,9 +119,9 @@ DNS_SCHED(48, 12, 4, 4, 3)
 DNS_SCHED(54, 6, 2, 3, 3, 3)
 DNS_SCHED(64, 8, 8, 8)
 DNS_SCHED(72, 12, 4, 2, 3, 3)
-DNS_SCHED(96, 12, 4, 4, 2, 3)
+DNS_SCHED(96, 12, 4, 2, 4, 3)
 DNS_SCHED(108, 12, 4, 3, 3, 3)
-DNS_SCHED(128, 8, 8, 4, 4)
+DNS_SCHED(128, 8, 8, 8, 2)
 DNS_SCHED(144, 12, 4, 4, 3, 3)
 DNS_SCHED(162, 6, 2, 3, 3, 3, 3)
 DNS_SCHED(192, 12, 4, 4, 3, 4)

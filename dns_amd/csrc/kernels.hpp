@@ -10,9 +10,9 @@
 //                     j*n+i (y slowest), element k at srow(row, k): only
 //                     the x pass touches the state during a step, and in
 //                     this layout its x-pencils have the short row stride of
-//                     a y pass (float x pass 2.72 -> 2.39 ms at n=512); at
-//                     n=1024 rows i, i+1 are interleaved so that a tile of
-//                     4 double columns covers whole 128-byte lines;
+//                     a y pass (float x pass 2.72 -> 2.39 ms at n=512), and
+//                     rows i, i+1 are interleaved so that a tile of 4
+//                     double columns covers whole 128-byte lines;
 //   physical field  : real [n][n][n], (i*n+j)*n+k.
 // A "pencil" is the set of elements along one axis with the other two
 // indices fixed.  x and y pencils are strided: a block owns T pencils with
@@ -51,7 +51,6 @@ struct Geo {
   // with one spare row 5.1 TB/s); wfs = n*wps per field.  Scratch uses of
   // the W buffer (transforms of the state, snapshots) keep the fs layout.
   long wfs, wps;
-  int spair;  // spectral state rows interleaved in pairs (srow)
 };
 
 // z pencils per block of the contiguous-axis passes: as many as fill one
@@ -118,20 +117,19 @@ __device__ __forceinline__ void state_row(long r, int n, int &i, int &j) {
 }
 
 // Offset of element k of row r in a spectral state field (uhn, uhr, acc,
-// phat and the adjoint's state-shaped arrays): r*nfp + k, or, where g.spair
-// is set, with rows 2q and 2q+1 (x rows i, i+1 of one y plane j, n being
-// even) interleaved in groups of four complex elements: row r's group m at
-// (r & ~1)*nfp + 8m + 4(r & 1).  A 128-byte line of double elements then
-// holds columns 4m..4m+3 of two x rows (of float elements 4m..4m+7), so an
-// x-pass tile of 4 double columns reads and writes whole lines; with one
-// row per line each line is split between two neighbouring tiles' blocks.
-// The engine sets spair where a pencil takes more than 64 threads (n=1024:
-// 128, so a 512-thread block holds 4 double columns): there the x pass
-// measured 39.5 -> 30.5 ms (FP64) and 24.1 -> 21.0 ms (mixed).  With 8-
-// column tiles (n=512) the interleaved rows measured slower (x pass 3.40 ->
-// 3.80 ms), so smaller grids keep one row per line.  nfp is a multiple of 4.
+// phat and the adjoint's state-shaped arrays).  Rows 2q and 2q+1 (x rows
+// i, i+1 of one y plane j, n being even) are interleaved in groups of four
+// complex elements: row r's group m sits at (r & ~1)*nfp + 8m + 4(r & 1).
+// A 128-byte line of double elements thus holds columns 4m..4m+3 of two x
+// rows (of float elements, 4m..4m+7), so an x-pass tile of 4 double
+// columns (n=1024: a pencil takes 128 threads, a 512-thread block holds 4)
+// reads and writes whole lines, where with one row per line each line was
+// split between two neighbouring tiles' blocks: x pass 39.5 -> 30.9 ms at
+// n=1024.  Wider tiles gain too: at n=512 (same-box runs taking turns) the
+// FP64 x pass 3.91 -> 3.87 ms, the float one 2.36 -> 2.31 ms.  nfp is a
+// multiple of 4.
 __device__ __forceinline__ long srow(long r, int k, const Geo &g) {
-  return g.spair ? (r & ~1L) * g.nfp + ((k >> 2) << 3) + ((r & 1) << 2) + (k & 3) : r * g.nfp + k;
+  return (r & ~1L) * g.nfp + ((k >> 2) << 3) + ((r & 1) << 2) + (k & 3);
 }
 
 // Converts `nfields` padded spectral fields between the x-major layout of
@@ -362,9 +360,9 @@ __global__ __launch_bounds__(T *(N / E), MINW) void k_xinv_curl(const cx<SI> *__
 #pragma unroll
   for (int c = 0; c < E; c++) {
     const long l = srow((long)j * N + t + TPP * c, k, g);
-    U[c] = ok ? cvt<R, SI>(ld_state<T>(uh + l, g.spair)) : zero<R>();
-    V[c] = ok ? cvt<R, SI>(ld_state<T>(uh + g.fs + l, g.spair)) : zero<R>();
-    Wv[c] = ok ? cvt<R, SI>(ld_state<T>(uh + 2 * g.fs + l, g.spair)) : zero<R>();
+    U[c] = ok ? cvt<R, SI>(ld_state<T>(uh + l)) : zero<R>();
+    V[c] = ok ? cvt<R, SI>(ld_state<T>(uh + g.fs + l)) : zero<R>();
+    Wv[c] = ok ? cvt<R, SI>(ld_state<T>(uh + 2 * g.fs + l)) : zero<R>();
   }
   curl_inverse_x<N, R, T, WITH_U, E>(U, V, Wv, W, g, t, p, j, k, ok, lds, tw);
 }
@@ -782,17 +780,17 @@ __device__ __forceinline__ void forward_x_rk(const cx<R> *__restrict__ W, cx<SR>
 #pragma unroll
       for (int f = 0; f < 3; f++) {
         const long lf = f * g.fs + srow(jrow + t + TPP * (c0 + i), k, g);
-        if constexpr (STAGE == 0 && MIXED_STATE) B.ur[f][i] = ld_state<T>(inc + lf, g.spair);
-        else if constexpr (STAGE == 0) B.ur[f][i] = ld_state<T>(uhn + lf, g.spair);
-        else B.ur[f][i] = ld_state<T>(uhr + lf, g.spair);
+        if constexpr (STAGE == 0 && MIXED_STATE) B.ur[f][i] = ld_state<T>(inc + lf);
+        else if constexpr (STAGE == 0) B.ur[f][i] = ld_state<T>(uhn + lf);
+        else B.ur[f][i] = ld_state<T>(uhr + lf);
         if constexpr (MIXED_STATE) {
-          if constexpr (STAGE == 1 || STAGE == 2) B.unf[f][i] = ld_state<T>(inc + lf, g.spair);
-          if constexpr (STAGE == 3) B.un[f][i] = ld_state<T>(uhn + lf, g.spair);
-          if constexpr (STAGE > 0 && S64) B.sf[f][i] = ld_state<T>(s64 + lf, g.spair);
-          else if constexpr (STAGE > 0) B.sf[f][i] = ld_state<T>(inc + 3 * g.fs + lf, g.spair);
+          if constexpr (STAGE == 1 || STAGE == 2) B.unf[f][i] = ld_state<T>(inc + lf);
+          if constexpr (STAGE == 3) B.un[f][i] = ld_state<T>(uhn + lf);
+          if constexpr (STAGE > 0 && S64) B.sf[f][i] = ld_state<T>(s64 + lf);
+          else if constexpr (STAGE > 0) B.sf[f][i] = ld_state<T>(inc + 3 * g.fs + lf);
         } else {
-          if constexpr (STAGE == 1 || STAGE == 2) B.un[f][i] = ld_state<T>(uhn + lf, g.spair);
-          if constexpr (STAGE > 0) B.ac[f][i] = ld_state<T>(acc + lf, g.spair);
+          if constexpr (STAGE == 1 || STAGE == 2) B.un[f][i] = ld_state<T>(uhn + lf);
+          if constexpr (STAGE > 0) B.ac[f][i] = ld_state<T>(acc + lf);
         }
       }
     }

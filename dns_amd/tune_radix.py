@@ -8,11 +8,13 @@ dividing the table's E, product n) with the fewest stages: each stage is
 one exchange through LDS.  Each candidate is compiled into a library of that
 one size, with the table's schedule replaced through -DDNS_SCHED_N,
 -DDNS_SCHED_E and -DDNS_SCHED_R, and timed on the GPU against the table's
-schedule (FP64 and mixed Taylor-Green steps, each library's launch shapes
-autotuned).  Every candidate must also reproduce the table schedule's
-result to rounding (energy and enstrophy after the timed steps within
-1e-12 relative; the spectral state within 1e-11 of its max norm up to
-n=256).  The chosen schedules are entered in the table by hand.
+schedule (FP64 and mixed Taylor-Green steps, default launch shape), each
+in several fresh processes taking turns, since a library's speed varies by
+a few percent between processes with where its buffers land in device
+memory.  Every candidate must also reproduce the table schedule's result to
+rounding (energy and enstrophy after the timed steps within 1e-12
+relative; the spectral state within 1e-11 of its max norm up to n=256).
+The chosen schedules are entered in the table by hand.
 
     python -m dns_amd.tune_radix build 128 256     # on the CPU: dns_amd/_tune/
     python -m dns_amd.tune_radix run 128 256       # on the GPU: one JSON line per candidate
@@ -124,6 +126,9 @@ def _tgv(n):
 
 
 def _time(path, n, prec, u, steps, want_state):
+    """One candidate library in this process: steps/s (best of three timed
+    runs of `steps` steps, default launch shape), energy and enstrophy after
+    them, and the spectral state if asked."""
     import numpy as np
     L = ctypes.CDLL(path)
     P, I, D, Lg = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_long
@@ -131,16 +136,12 @@ def _time(path, n, prec, u, steps, want_state):
     L.dns_load_velocity.argtypes = [P, P]
     L.dns_step.argtypes = [P, Lg]
     L.dns_diagnostics.argtypes = [P, ctypes.POINTER(D), ctypes.POINTER(D)]
-    L.dns_autotune.argtypes = [P, I, ctypes.POINTER(I)]
     L.dns_get_spectral.argtypes = [P, P]
     L.dns_destroy.argtypes = [P]
     ctx = P()
     nu, dt = (1 / 3000, 5e-4) if n == 512 else (1 / 1600, 1e-3)
     assert L.dns_create(ctypes.byref(ctx), n, nu, dt, prec, 0) == 0
     try:
-        assert L.dns_load_velocity(ctx, u.ctypes.data) == 0
-        shape = I(0)
-        assert L.dns_autotune(ctx, 2, ctypes.byref(shape)) == 0
         assert L.dns_load_velocity(ctx, u.ctypes.data) == 0
         e, z = D(), D()
         assert L.dns_step(ctx, 3) == 0
@@ -156,37 +157,78 @@ def _time(path, n, prec, u, steps, want_state):
             nf = n // 2 + 1
             state = np.empty((3, n, n, nf, 2))
             assert L.dns_get_spectral(ctx, state.ctypes.data) == 0
-        return {"steps_per_s": steps / best, "E": e.value, "Z": z.value, "shape": shape.value}, state
+        return {"steps_per_s": steps / best, "E": e.value, "Z": z.value}, state
     finally:
         L.dns_destroy(ctx)
 
 
-def run(sizes, out):
+def _one(path, n, prec, steps, state_out):
+    """Child process: time one candidate library, print its result line."""
+    import numpy as np
+    r, st = _time(path, n, prec, _tgv(n), steps, bool(state_out))
+    if st is not None:
+        np.save(state_out, st)
+    print("RESULT " + json.dumps(r), flush=True)
+
+
+def run(sizes, out, repeats=3, precisions=("fp64", "mixed")):
+    """Times every built candidate of each size.  The speed of one library
+    varies by a few percent from process to process with where its buffers
+    land in device memory, so each candidate runs in `repeats` fresh
+    processes, the candidates taking turns (rotated order per repeat), and
+    the median counts.  The first repeat also checks each candidate against
+    the table's schedule: energy and enstrophy after the timed steps (FP64
+    within 1e-12, mixed 1e-5) and, up to n=256, the spectral state (1e-11,
+    mixed 1e-4, of its max norm)."""
+    import statistics
+    import subprocess
+    import tempfile
     import numpy as np
     res = []
+    tmp = tempfile.mkdtemp(prefix="tune_radix_")
     for n in sizes:
-        u = _tgv(n)
         steps = max(6, int(2e6 / n ** 2))
         e, cands = candidates(n)
-        for prec, pname in ((0, "fp64"), (1, "mixed")):
-            ref = None
+        for pname in precisions:
+            prec = {"fp64": 0, "mixed": 1}[pname]
+            runs = {c: [] for c in cands}
+            check = {}
+            for rep in range(repeats):
+                order = cands[rep % len(cands):] + cands[:rep % len(cands)]
+                for rad in order:
+                    path = os.path.join(TUNE, name(n, rad), "libdns_hip.so")
+                    st_out = os.path.join(tmp, name(n, rad) + ".npy") if rep == 0 and n <= 256 else ""
+                    p = subprocess.run([sys.executable, "-m", "dns_amd.tune_radix", "one", path, str(n), str(prec),
+                                        str(steps), st_out or "-"], capture_output=True, text=True, timeout=900,
+                                       cwd=ROOT)
+                    line = [l for l in p.stdout.splitlines() if l.startswith("RESULT ")]
+                    if p.returncode or not line:
+                        raise RuntimeError(f"candidate {name(n, rad)} {pname} failed:\n{p.stderr[-2000:]}")
+                    r = json.loads(line[0][7:])
+                    runs[rad].append(r["steps_per_s"])
+                    if rep == 0:
+                        check[rad] = (r, st_out)
+                    print(f"n={n} {pname} {'-'.join(map(str, rad))} rep {rep}: {r['steps_per_s']:.2f} steps/s",
+                          flush=True)
+            r0, st0 = check[cands[0]]
+            ref_state = np.load(st0) if st0 else None
+            med0 = statistics.median(runs[cands[0]])
             for rad in cands:
-                path = os.path.join(TUNE, name(n, rad), "libdns_hip.so")
-                r, st = _time(path, n, prec, u, steps, n <= 256)
-                r.update({"n": n, "precision": pname, "E_per_thread": e, "radices": list(rad), "steps": steps,
-                          "table": rad == cands[0]})
-                if ref is None:
-                    ref = (r, st)
-                else:
-                    r["dE"] = abs(r["E"] - ref[0]["E"]) / ref[0]["E"]
-                    r["dZ"] = abs(r["Z"] - ref[0]["Z"]) / ref[0]["Z"]
-                    if st is not None:
-                        r["dstate"] = float(np.max(np.abs(st - ref[1])) / np.max(np.abs(ref[1])))
-                    r["speedup_vs_table"] = r["steps_per_s"] / ref[0]["steps_per_s"]
-                    tol = 1e-12 if prec == 0 else 1e-5
-                    r["ok"] = r["dE"] <= tol and r["dZ"] <= tol and r.get("dstate", 0.0) <= (1e-11 if prec == 0 else 1e-4)
-                print(json.dumps(r), flush=True)
-                res.append(r)
+                r, st = check[rad]
+                med = statistics.median(runs[rad])
+                row = {"n": n, "precision": pname, "E_per_thread": e, "radices": list(rad), "steps": steps,
+                       "table": rad == cands[0], "steps_per_s": runs[rad], "median": med,
+                       "speedup_vs_table": med / med0,
+                       "dE": abs(r["E"] - r0["E"]) / r0["E"], "dZ": abs(r["Z"] - r0["Z"]) / r0["Z"]}
+                if ref_state is not None:
+                    row["dstate"] = float(np.max(np.abs(np.load(st) - ref_state)) / np.max(np.abs(ref_state)))
+                tol, stol = (1e-12, 1e-11) if prec == 0 else (1e-5, 1e-4)
+                row["ok"] = row["dE"] <= tol and row["dZ"] <= tol and row.get("dstate", 0.0) <= stol
+                print(json.dumps(row), flush=True)
+                res.append(row)
+            for rad in cands:
+                if check[rad][1]:
+                    os.remove(check[rad][1])
     if out:
         with open(out, "w") as f:
             json.dump(res, f, indent=1)
@@ -194,9 +236,16 @@ def run(sizes, out):
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    if argv is None:
+        argv = sys.argv[1:]
+    if argv and argv[0] == "one":  # child process of `run`
+        path, n, prec, steps, st = argv[1:6]
+        return _one(path, int(n), int(prec), int(steps), "" if st == "-" else st)
     ap.add_argument("what", choices=["list", "build", "run"])
     ap.add_argument("sizes", type=int, nargs="+")
     ap.add_argument("-o", "--out", default=None)
+    ap.add_argument("--repeats", type=int, default=3, help="fresh processes per candidate (run)")
+    ap.add_argument("--precisions", default="fp64,mixed", help="comma-separated: fp64, mixed (run)")
     a = ap.parse_args(argv)
     if a.what == "list":
         for n in a.sizes:
@@ -205,7 +254,7 @@ def main(argv=None):
     elif a.what == "build":
         build(a.sizes)
     else:
-        run(a.sizes, a.out)
+        run(a.sizes, a.out, a.repeats, tuple(a.precisions.split(",")))
 
 
 if __name__ == "__main__":
