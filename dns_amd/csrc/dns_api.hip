@@ -4,11 +4,35 @@
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
+#include <cstdlib>
+#include <dlfcn.h>
 #include <new>
 #include "engine.hpp"
 #include "grid_sizes.h"
 
 namespace dnsk {
+
+// roctx entry points, opened once on first use when DNS_ROCTX is set (and
+// not "0"); a missing library leaves the ranges off.
+const RoctxApi &roctx_api() {
+  static const RoctxApi api = [] {
+    RoctxApi a;
+    const char *e = getenv("DNS_ROCTX");
+    if (!e || !*e || strcmp(e, "0") == 0) return a;
+    const char *libs[] = {"librocprofiler-sdk-roctx.so.1", "libroctx64.so.4"};
+    for (const char *l : libs) {
+      void *h = dlopen(l, RTLD_NOW | RTLD_LOCAL);
+      if (!h) continue;
+      a.push = (int (*)(const char *))dlsym(h, "roctxRangePushA");
+      a.pop = (int (*)())dlsym(h, "roctxRangePop");
+      if (a.push && a.pop) break;
+      a.push = nullptr;
+      a.pop = nullptr;
+    }
+    return a;
+  }();
+  return api;
+}
 
 static thread_local char g_err[512];
 
@@ -175,39 +199,49 @@ static int ctx_autotune(dns_ctx *c, int steps, int *best_out, double *best_ms) {
     const size_t sb = (size_t)c->members * c->g.mS * sizeof(double2);
     const size_t ub = (size_t)c->members * c->g.mR * c->impl->r_size();
     void *su = nullptr, *sus = nullptr;
-    DNS_HIP(hipMalloc(&su, sb));
-    if (hipMalloc(&sus, ub) != hipSuccess) {
-      hipFree(su);
-      return DNS_ENOMEM;
-    }
-    hipEvent_t a, b;
-    hipEventCreate(&a);
-    hipEventCreate(&b);
+    hipEvent_t a = nullptr, b = nullptr;
     int rc = DNS_OK;
-    hipMemcpyAsync(su, c->uhn, sb, hipMemcpyDeviceToDevice, c->stream);
-    hipMemcpyAsync(sus, c->us, ub, hipMemcpyDeviceToDevice, c->stream);
+    // keeps the first failure; the state is restored whatever fails later
+    auto hip = [&](hipError_t e, const char *what) {
+      if (e != hipSuccess && rc == DNS_OK) {
+        set_error(what, e);
+        rc = e == hipErrorOutOfMemory ? DNS_ENOMEM : DNS_EHIP;
+      }
+    };
+    hip(hipMalloc(&su, sb), "hipMalloc");
+    if (rc == DNS_OK) hip(hipMalloc(&sus, ub), "hipMalloc");
+    if (rc == DNS_OK) hip(hipEventCreate(&a), "hipEventCreate");
+    if (rc == DNS_OK) hip(hipEventCreate(&b), "hipEventCreate");
+    const bool saved = rc == DNS_OK;
+    if (saved) {
+      hip(hipMemcpyAsync(su, c->uhn, sb, hipMemcpyDeviceToDevice, c->stream), "hipMemcpyAsync");
+      hip(hipMemcpyAsync(sus, c->us, ub, hipMemcpyDeviceToDevice, c->stream), "hipMemcpyAsync");
+    }
     for (int v = 0; v < nv && rc == DNS_OK; v++) {
       c->impl->set_variant(c, v);
       rc = c->impl->step(c, 1);
-      hipEventRecord(a, c->stream);
+      if (rc == DNS_OK) hip(hipEventRecord(a, c->stream), "hipEventRecord");
       if (rc == DNS_OK) rc = c->impl->step(c, steps);
-      hipEventRecord(b, c->stream);
-      hipEventSynchronize(b);
+      if (rc == DNS_OK) hip(hipEventRecord(b, c->stream), "hipEventRecord");
+      if (rc == DNS_OK) hip(hipEventSynchronize(b), "hipEventSynchronize");
       float ms = 0;
-      hipEventElapsedTime(&ms, a, b);
-      if (v == 0 || ms < bms) {
+      if (rc == DNS_OK) hip(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
+      if (rc == DNS_OK && (v == 0 || ms < bms)) {
         bms = ms;
         best = v;
       }
     }
-    hipMemcpyAsync(c->uhn, su, sb, hipMemcpyDeviceToDevice, c->stream);
-    hipMemcpyAsync(c->us, sus, ub, hipMemcpyDeviceToDevice, c->stream);
-    c->impl->state_changed(c, -1);
-    hipStreamSynchronize(c->stream);
-    hipEventDestroy(a);
-    hipEventDestroy(b);
-    hipFree(su);
-    hipFree(sus);
+    if (saved) {
+      hip(hipMemcpyAsync(c->uhn, su, sb, hipMemcpyDeviceToDevice, c->stream), "hipMemcpyAsync");
+      hip(hipMemcpyAsync(c->us, sus, ub, hipMemcpyDeviceToDevice, c->stream), "hipMemcpyAsync");
+      const int sc = c->impl->state_changed(c, -1);
+      if (rc == DNS_OK) rc = sc;
+      hip(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    }
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    if (su) (void)hipFree(su);
+    if (sus) (void)hipFree(sus);
     c->w_ready = 0;
     c->have_phat = 0;
     if (rc) return rc;
@@ -260,27 +294,27 @@ int dns_create(dns_ctx **out, int n, double nu, double dt, int precision, int de
 
 void dns_destroy(dns_ctx *c) {
   if (!c) return;
-  hipSetDevice(c->device);
-  if (c->stream) hipStreamSynchronize(c->stream);
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
   void *ptrs[] = {c->uhn, c->uhr, c->acc, c->W, c->us, c->phat, c->tw64, c->tw32, c->part, c->dres, c->nudt};
   for (void *p : ptrs)
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
   for (auto &t : c->ev_pending) {
     c->ev_pool.push_back(t.a);
     c->ev_pool.push_back(t.b);
   }
   for (hipEvent_t e : c->ev_pool)
-    if (e) hipEventDestroy(e);
+    if (e) (void)hipEventDestroy(e);
   delete (AdjWork *)c->adj;
-  if (c->snap_stream) hipStreamSynchronize(c->snap_stream);
-  if (c->snap_dev) hipFree(c->snap_dev);
-  if (c->snap_host) hipHostFree(c->snap_host);
-  if (c->snap_ready) hipEventDestroy(c->snap_ready);
-  if (c->snap_done) hipEventDestroy(c->snap_done);
-  if (c->snap_stream) hipStreamDestroy(c->snap_stream);
-  if (c->gexec) hipGraphExecDestroy(c->gexec);
-  if (c->gexec1) hipGraphExecDestroy(c->gexec1);
-  if (c->stream) hipStreamDestroy(c->stream);
+  if (c->snap_stream) (void)hipStreamSynchronize(c->snap_stream);
+  if (c->snap_dev) (void)hipFree(c->snap_dev);
+  if (c->snap_host) (void)hipHostFree(c->snap_host);
+  if (c->snap_ready) (void)hipEventDestroy(c->snap_ready);
+  if (c->snap_done) (void)hipEventDestroy(c->snap_done);
+  if (c->snap_stream) (void)hipStreamDestroy(c->snap_stream);
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->gexec1) (void)hipGraphExecDestroy(c->gexec1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c->impl;
   delete c;
 }
@@ -487,14 +521,14 @@ int dns_irfft3(dns_ctx *c, const double *spec, double *real) {
 
 void dns_batch_destroy(dns_batch *b) {
   if (!b) return;
-  hipSetDevice(b->device);
-  if (b->exec) hipGraphExecDestroy(b->exec);
-  if (b->exec1) hipGraphExecDestroy(b->exec1);
-  if (b->main) hipStreamSynchronize(b->main);
+  (void)hipSetDevice(b->device);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->exec1) (void)hipGraphExecDestroy(b->exec1);
+  if (b->main) (void)hipStreamSynchronize(b->main);
   for (dns_ctx *c : b->groups) dns_destroy(c);
-  for (hipEvent_t e : b->join) hipEventDestroy(e);
-  if (b->fork) hipEventDestroy(b->fork);
-  if (b->main) hipStreamDestroy(b->main);
+  for (hipEvent_t e : b->join) (void)hipEventDestroy(e);
+  if (b->fork) (void)hipEventDestroy(b->fork);
+  if (b->main) (void)hipStreamDestroy(b->main);
   delete b;
 }
 

@@ -158,6 +158,28 @@ inline void t_end(dns_ctx *c) {
   hipEvent_t b = c->ev_pending.back().b;
   if (b) (void)hipEventRecord(b, c->stream);
 }
+// roctx ranges (rocprofv3 --marker-trace) around every step and RK stage
+// the host launches, when the environment sets DNS_ROCTX: the roctx
+// library is opened at run time (roctx_api in dns_api.hip), so the library
+// has no link dependency on it.  Under a captured graph the ranges mark the
+// capture, not the replays.
+struct RoctxApi {
+  int (*push)(const char *) = nullptr;
+  int (*pop)() = nullptr;
+};
+const RoctxApi &roctx_api();
+struct RoctxRange {
+  bool on;
+  explicit RoctxRange(const char *name) : on(roctx_api().push != nullptr) {
+    if (on) roctx_api().push(name);
+  }
+  ~RoctxRange() {
+    if (on) roctx_api().pop();
+  }
+  RoctxRange(const RoctxRange &) = delete;
+  RoctxRange &operator=(const RoctxRange &) = delete;
+};
+
 #define DNS_TIMED(c, slot, launch) \
   do {                             \
     t_begin(c, slot);              \
@@ -472,9 +494,12 @@ template <int N, typename R, typename SR, bool S64 = false> struct Engine final 
     const dim3 gy_trunc = PYF ? dim3(paired_blocks((g.nk + TYF - 1) / TYF, N), 1, M)
                               : dim3((g.nk + TYF - 1) / TYF, N, M),
                byf(TYF * TPP_Y);
+    static const char *const stage_names[4] = {"rk stage 1", "rk stage 2", "rk stage 3", "rk stage 4"};
     for (long st = 0; st < nsteps; st++) {
+      const RoctxRange step_range("dns step");
       double2 *ph = (c->capture && st == nsteps - 1) ? (double2 *)c->phat : nullptr;
       for (int rk = 0; rk < 4; rk++) {
+        const RoctxRange stage_range(stage_names[rk]);
         RKParams q{c->nudt, c->kmax, a_rk[rk], b_rk[rk]};
         const int sl = DNS_NPASS * rk;
         // pass A of this stage ran fused into the previous stage's E
