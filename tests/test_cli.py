@@ -37,6 +37,19 @@ def test_argument_errors(exe, args, msg):
     assert r.stdout == ""
 
 
+def test_unknown_precision_is_refused(tmp_path):
+    """DNS_PRECISION (fp64, mixed, mixed_acc64) is checked before any device
+    call; any other value ends the run with an error line."""
+    p = tmp_path / "tgv.raw"
+    write_raw(p, tgv_field(8)[:3])
+    env = dict(os.environ, DNS_PRECISION="half")
+    r = subprocess.run([DNS_CLI, "-t", "1", "-n", "0.01", "-s", "0.1", "-i", str(p)], capture_output=True,
+                       text=True, env=env)
+    assert r.returncode == 1
+    assert r.stderr == "dns: error: DNS_PRECISION must be fp64, mixed or mixed_acc64\n"
+    assert r.stdout == ""
+
+
 @pytest.mark.parametrize("exe", [DNS_CLI, ORACLE_CLI], ids=["dns", "dns_cpu"])
 def test_wrong_file_size(tmp_path, exe):
     p = tmp_path / "bad.raw"

@@ -47,12 +47,14 @@ def test_config0_tgv64_1000_steps(tmp_path):
     _compare(g, o)
 
 
-def test_mixed_cli_within_budget(tmp_path):
+@pytest.mark.parametrize("precision", ["mixed", "mixed_acc64"])
+def test_mixed_cli_within_budget(tmp_path, precision):
     raw = tmp_path / "tgv64.raw"
     _run(DNS_IC, ["-l", "6", "-o", str(raw)], tmp_path)
     args = ["-i", str(raw), "-n", "0.01", "-s", "0.01", "-t", "1.995"]
     a = parse_lines(_run(DNS_CLI, args, tmp_path).stdout)
-    b = parse_lines(_run(DNS_CLI, args, tmp_path, {"DNS_PRECISION": "mixed"}).stdout)
+    b = parse_lines(_run(DNS_CLI, args, tmp_path, {"DNS_PRECISION": precision}).stdout)
+    assert [x[0] for x in a] == [x[0] for x in b]
     for (s1, _, e1, _z1), (s2, _, e2, _z2) in zip(a, b):
         assert abs(e1 - e2) / e1 < 1e-5
 
@@ -161,7 +163,7 @@ def test_nonfinite_input_raises_sigfpe(tmp_path):
     assert g.stdout == o.stdout == ""
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "mixed_acc64"])
 def test_unstable_run_raises_sigfpe(tmp_path, precision):
     """dt = 5 at n=16 overflows by step 3: the same stdout (the line of
     step 0) and the same signal as the oracle, whose traps fire at the
