@@ -270,10 +270,12 @@ template <int N, typename R, typename SR, bool S64 = false> struct Engine final 
   static constexpr int NF = N / 2 + 1;
   // recording kernels exist for FP64 only (the adjoint's precision)
   static constexpr bool REC_OK = sizeof(R) == 8;
-  // minimum waves per SIMD the z pass's registers must allow (n=1024 FP64:
-  // 2 instead of 1 spills 92-124 bytes per lane in stages 2-4 and measured
-  // no faster, 20.4 against 20.0-20.2 ms per launch in same-box runs)
-  static constexpr int MINW_Z = 1;
+  // minimum waves per SIMD the z pass's registers must allow.  Float with a
+  // pencil in at most one wave: 4 (at most 128 VGPRs instead of 136-152),
+  // 0.982 -> 0.911 ms per launch at n=512 and 0.128 -> 0.119 ms at n=256 in
+  // alternating same-box runs.  FP64 keeps 1 (n=1024: 2 spills 92-124 bytes
+  // per lane in stages 2-4 and measured no faster, 20.4 against 20.0-20.2 ms).
+  static constexpr int MINW_Z = (sizeof(R) == 4 && TPP <= 64) ? 4 : 1;
 
   size_t sr_size() const override { return sizeof(cx<SR>); }
   // bytes per member of the accumulator region: 3 double fields, or in
