@@ -222,6 +222,29 @@ def beltrami_field(n, k=1, a=1.0, b=0.7, c=0.4):
                      c * np.sin(k * Y) + b * np.cos(k * X)])
 
 
+def helical_spectrum(n, ks, seed):
+    """Unnormalised half spectrum [3][n][n][n/2+1] (numpy rfftn convention)
+    of u = sum over ks of Re(A h e^{i k.x}): h the eigenvector of the
+    Fourier curl i k x . with eigenvalue +|k|, A a seeded complex amplitude.
+    Every k has kz > 0 and the same |k|, so curl u = |k| u (a Beltrami field
+    with modes at many kz).  The array is allocated lazily (np.zeros): only
+    the pages of the modes are written."""
+    ks = [tuple(int(v) for v in k) for k in ks]
+    shells = {k[0] ** 2 + k[1] ** 2 + k[2] ** 2 for k in ks}
+    assert len(shells) == 1 and all(0 < k[2] < n // 2 for k in ks) and len(set(ks)) == len(ks)
+    kk = shells.pop()
+    rng = np.random.default_rng(seed)
+    spec = np.zeros((3, n, n, n // 2 + 1), dtype=np.complex128)
+    for k in ks:
+        kx, ky, kz = k
+        curl = 1j * np.array([[0, -kz, ky], [kz, 0, -kx], [-ky, kx, 0]], dtype=float)
+        w, v = np.linalg.eig(curl)
+        h = v[:, int(np.argmin(np.abs(w - math.sqrt(kk))))]
+        a = (rng.uniform(0.5, 1.0) * np.exp(2j * np.pi * rng.uniform())) * h / np.linalg.norm(h)
+        spec[:, kx % n, ky % n, kz] = n ** 3 * a / 2
+    return spec, kk
+
+
 def wavenumbers(n):
     """kx, ky, kz of the half-spectrum layout [n][n][n/2+1] (Nyquist of x
     and y at -n/2, main.c:224-230)."""

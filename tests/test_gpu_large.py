@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 
 import dns_amd
-from conftest import beltrami_field, rel, rk4_amplification
+from conftest import beltrami_field, helical_spectrum, rel, rk4_amplification
 
 pytestmark = pytest.mark.gpu
 
@@ -135,3 +135,53 @@ def test_beltrami_n1024(precision, tol):
         e, z = s.diagnostics()
         f = rk4_amplification(nu * dt * k * k) ** (2 * m)
         assert abs(e - e0 * f) / (e0 * f) < tol and abs(z - z0 * f) / (z0 * f) < tol
+
+
+# |k|^2 = 25625 = 160^2 + 5^2 = 128^2 + 96^2 + 5^2 = 155^2 + 40^2
+HELICAL = [(3, 4, 160), (-4, 3, 160), (5, 0, 160), (128, 5, 96), (-5, 128, 96), (128, 96, 5),
+                (-96, 128, 5), (160, 3, 4), (4, 160, 3), (96, 5, 128), (155, 0, 40), (0, 155, 40)]
+
+
+@pytest.mark.parametrize("n,precision,tol", [(1024, "fp64", 1e-12), (1024, "mixed", 1e-5), (972, "fp64", 1e-12)])
+def test_helical_beltrami(n, precision, tol):
+    """A Beltrami field of twelve helical modes with |k|^2 = 25625 at kz = 3,
+    4, 5, 40, 96, 128 and 160 (the ABC fields above have kz = 0 and 9 only).
+    The z pencils of n=1024 (128 threads) and n=972 (81 threads) span two
+    waves and rebuild the mirrored half of each pencil from these rows, in
+    threads of both waves (positions n - kz), and u x omega vanishes only if
+    every point is right: then every coefficient of the state is multiplied
+    by the RK4 polynomial R(nu dt |k|^2) per step.  The time step keeps the
+    advective number kmax max|u| dt (about 342 * 5.3 * 0.001 = 1.8) inside
+    RK4's stability bound of about 2.8: at dt = 0.01 (about 18) rounding
+    noise at the dealias edge grows by orders of magnitude per step, in FP64
+    (from 1e-16) as in mixed precision (from 1e-7, blowing up by step 4)."""
+    nu, dt, m = 2e-4, 1e-3, 3
+    print(f"helical beltrami {precision} n={n}", flush=True)
+    spec, kk = helical_spectrum(n, HELICAL, seed=1024)
+    # the physical field through the library's plain FP64 c2r, not the fused
+    # z pass (a float c2r would leave rounding noise whose curl, at high
+    # wavenumbers, no longer cancels in u x omega)
+    u = np.empty((3, n, n, n))
+    with dns_amd.Solver(n, nu, dt, "fp64") as s:
+        for c in range(3):
+            u[c] = s.irfft3(spec[c])
+    del spec
+    u /= float(n) ** 3
+    with dns_amd.Solver(n, nu, dt, precision) as s:
+        s.load_velocity(u)
+        del u
+        a0 = s.get_spectral()
+        e0, z0 = s.diagnostics()
+        assert abs(z0 - kk * e0) <= 1e-12 * z0
+        s.step(m)
+        e, z = s.diagnostics()
+        f = rk4_amplification(nu * dt * kk)
+        a = s.get_spectral()
+        errs = []
+        for c in range(3):
+            a0[c] *= f ** m
+            errs.append(rel(a[c], a0[c]))
+        de = abs(e - e0 * f ** (2 * m)) / (e0 * f ** (2 * m))
+        dz = abs(z - z0 * f ** (2 * m)) / (z0 * f ** (2 * m))
+        print(f"state {max(errs):.3e} energy {de:.3e} enstrophy {dz:.3e}", flush=True)
+        assert max(errs) < tol and de < tol and dz < tol
