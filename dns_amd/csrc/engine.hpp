@@ -270,12 +270,15 @@ template <int N, typename R, typename SR, bool S64 = false> struct Engine final 
   static constexpr int NF = N / 2 + 1;
   // recording kernels exist for FP64 only (the adjoint's precision)
   static constexpr bool REC_OK = sizeof(R) == 8;
-  // minimum waves per SIMD the z pass's registers must allow.  Float with a
-  // pencil in at most one wave: 4 (at most 128 VGPRs instead of 136-152),
-  // 0.982 -> 0.911 ms per launch at n=512 and 0.128 -> 0.119 ms at n=256 in
-  // alternating same-box runs.  FP64 keeps 1 (n=1024: 2 spills 92-124 bytes
-  // per lane in stages 2-4 and measured no faster, 20.4 against 20.0-20.2 ms).
-  static constexpr int MINW_Z = (sizeof(R) == 4 && TPP <= 64) ? 4 : 1;
+  // minimum waves per SIMD the z pass's registers must allow.  Float with 8
+  // elements per thread and a pencil in at most one wave (n = 32..512): 4
+  // (at most 128 VGPRs instead of 136-152, spilling at most 28 bytes per
+  // lane), 0.982 -> 0.911 ms per launch at n=512 and 0.128 -> 0.119 ms at
+  // n=256 in alternating same-box runs.  Not with 12 or 6 elements per
+  // thread (the radix-3 grids), where that bound spills 84-616 and up to 72
+  // bytes per lane.  FP64 keeps 1 (n=1024: 2 spills 92-124 bytes per lane
+  // in stages 2-4 and measured no faster, 20.4 against 20.0-20.2 ms).
+  static constexpr int MINW_Z = (sizeof(R) == 4 && TPP <= 64 && E == 8) ? 4 : 1;
 
   size_t sr_size() const override { return sizeof(cx<SR>); }
   // bytes per member of the accumulator region: 3 double fields, or in
