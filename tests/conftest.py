@@ -222,9 +222,11 @@ def beltrami_field(n, k=1, a=1.0, b=0.7, c=0.4):
                      c * np.sin(k * Y) + b * np.cos(k * X)])
 
 
-def helical_spectrum(n, ks, seed):
-    """Unnormalised half spectrum [3][n][n][n/2+1] (numpy rfftn convention)
-    of u = sum over ks of Re(A h e^{i k.x}): h the eigenvector of the
+def helical_spectrum(n, ks, seed, scale=None):
+    """Unnormalised half spectrum [3][n][n][n/2+1] (numpy rfftn convention;
+    coefficient scale * A h / 2 at k, scale n^3 by default, 1 for a spectrum
+    whose unnormalised c2r is u itself) of u = sum over ks of
+    Re(A h e^{i k.x}): h the eigenvector of the
     Fourier curl i k x . with eigenvalue +|k|, A a seeded complex amplitude.
     Every k has kz > 0 and the same |k|, so curl u = |k| u (a Beltrami field
     with modes at many kz).  The array is allocated lazily (np.zeros): only
@@ -241,7 +243,7 @@ def helical_spectrum(n, ks, seed):
         w, v = np.linalg.eig(curl)
         h = v[:, int(np.argmin(np.abs(w - math.sqrt(kk))))]
         a = (rng.uniform(0.5, 1.0) * np.exp(2j * np.pi * rng.uniform())) * h / np.linalg.norm(h)
-        spec[:, kx % n, ky % n, kz] = n ** 3 * a / 2
+        spec[:, kx % n, ky % n, kz] = (float(n) ** 3 if scale is None else scale) * a / 2
     return spec, kk
 
 
